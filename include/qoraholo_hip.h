@@ -1,0 +1,219 @@
+/* C-ABI of the MI355X holotomography backend (Backend::hip of QoraHolo).
+ *
+ * Plain pointers, sizes and POD structs only. Every entry point returns a qh_status; on error
+ * qh_hip_last_error() holds a message for the calling thread. Host-array entry points copy in,
+ * compute on the device and copy out, as the reference's CUDA backend does per call; arguments
+ * are validated here with the reference's rules (src/api.cpp:13-60,107-123), so the host layer
+ * above may pass them through unchecked.
+ *
+ * Complex data are interleaved (re, im) float pairs: qh_c64. Arrays are row-major (z, y, x),
+ * x fastest, centre-origin in real space, as the reference's Array3 (include/qoraholo/array.hpp).
+ */
+#ifndef QORAHOLO_HIP_H
+#define QORAHOLO_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. The host layers rethrow them as the reference does: INVALID_ARGUMENT as
+ * std::invalid_argument (src/api.cpp:13-15), RUNTIME and OUT_OF_MEMORY as std::runtime_error
+ * (src/cuda/util.cuh:16-26), LOGIC as std::logic_error (src/pipeline.cpp:108). */
+typedef enum {
+  QH_OK = 0,
+  QH_ERR_INVALID_ARGUMENT = 1,
+  QH_ERR_RUNTIME = 2,
+  QH_ERR_OUT_OF_MEMORY = 3,
+  QH_ERR_LOGIC = 4
+} qh_status;
+
+typedef struct {
+  float re;
+  float im;
+} qh_c64;
+
+/* include/qoraholo/odt.hpp:13-18 (Geometry). Micrometres. */
+typedef struct {
+  float wavelength;
+  float n_medium;
+  float na;
+  float pixel;
+} qh_geometry;
+
+/* include/qoraholo/odt.hpp:22-25 (Illumination), cycles per micrometre. */
+typedef struct {
+  float fy;
+  float fx;
+} qh_illumination;
+
+/* src/backends.hpp:20-25 (CropPlan). */
+typedef struct {
+  int cy;
+  int cx;
+  float radius;
+  int out;
+} qh_crop_plan;
+
+/* src/common.hpp:36-44 (OdtGrid). */
+typedef struct {
+  int n;
+  int nz;
+  float df;
+  float dfz;
+  float fm2;
+  float fna2;
+  float gain;
+} qh_odt_grid;
+
+/* include/qoraholo/odt.hpp:46-57 (OdtOptions without the backend). solver: 0 Gerchberg-
+ * Papoulis, 1 FISTA-TV. gp_transform: 0 r2c, 1 c2c. mapping: 0 gather, 1 scatter; both run
+ * the deterministic gather here, which odt.hpp:26-29 allows. */
+typedef struct {
+  int nz;
+  int iterations;
+  int solver;
+  int gp_transform;
+  float tv_weight;
+  int tv_inner;
+  int mapping;
+} qh_odt_options;
+
+/* src/backends.hpp:31-42 (OdtTimings), device event intervals in milliseconds, plus the summed
+ * duration of the FGP dual-step kernels and their count (FISTA-TV only). */
+typedef struct {
+  size_t bytes;
+  double setup;
+  double upload;
+  double fft2;
+  double clear;
+  double mapping;
+  double normalize;
+  double solve;
+  double finish;
+  double download;
+  double fgp_ms;
+  int fgp_steps;
+} qh_odt_timings;
+
+/* include/qoraholo/pipeline.hpp:31-43 (PipelineTimings), plus the FGP kernel totals. */
+typedef struct {
+  double upload;
+  double fields;
+  double rytov;
+  double fft2;
+  double clear;
+  double mapping;
+  double normalize;
+  double solve;
+  double finish;
+  double download;
+  double total;
+  double fgp_ms;
+  int fgp_steps;
+} qh_pipeline_timings;
+
+/* Message of the last failed call on this thread ("" when none). */
+const char* qh_hip_last_error(void);
+
+/* src/backends.hpp:109 (cuda::device_present) and include/qoraholo/backend.hpp:8
+ * (cuda_available): 1 when a HIP device is visible. */
+int qh_hip_device_present(void);
+
+/* ---- host helpers the dispatch layer uses (no device work) ---- */
+
+/* src/api.cpp:32-42 (make_grid). */
+int qh_make_grid(int n, int nz, const qh_geometry* geometry, qh_odt_grid* out);
+
+/* src/api.cpp:52-60 (crop_plan): integer centre by lround, default radius |c| / 3, checks. */
+int qh_resolve_crop(float center_y, float center_x, float radius, int output_size, int n,
+                 qh_crop_plan* out);
+
+/* src/api.cpp:203-216 (circular_scan). */
+int qh_circular_scan(int count, float na_illumination, const qh_geometry* geometry,
+                     qh_illumination* out);
+
+/* The noise stream of src/phantom.cpp:200-201: std::mt19937_64(seed) drawn through
+ * std::normal_distribution<float>(0, sigma), `count` values in order. */
+int qh_gaussian_noise(uint64_t seed, float sigma, size_t count, float* out);
+
+/* ---- per-call entry points, host arrays (src/backends.hpp:112-118) ---- */
+
+/* cuda::extract_fields: holograms (count, n, n) -> fields (count, plan.out, plan.out). */
+int qh_hip_extract_fields(const float* holograms, int count, int n, const qh_crop_plan* plan,
+                          qh_c64* fields);
+
+/* cuda::unwrap: wrapped phase (nz, ny, nx) -> unwrapped phase. */
+int qh_hip_unwrap(const float* wrapped, int nz, int ny, int nx, float* out);
+
+/* cuda::rytov: field ratio (nz, ny, nx) -> Rytov field. */
+int qh_hip_rytov(const qh_c64* ratio, int nz, int ny, int nx, qh_c64* out);
+
+/* cuda::reconstruct: Rytov fields (count, n, n) -> index (nz, n, n) and the filled fraction.
+ * timings may be NULL. */
+int qh_hip_reconstruct(const qh_c64* fields, int count, int n, const qh_illumination* illum,
+                       const qh_odt_grid* grid, const qh_geometry* geometry,
+                       const qh_odt_options* options, float* ri, double* filled,
+                       qh_odt_timings* timings);
+
+/* The Ewald mapping alone (the first stage of cuda::reconstruct, src/cuda/odt.cu:243-262):
+ * normalised k-space potential (nz, n, n) and per-voxel hit counts, for checks of the
+ * intermediate. No counterpart entry in the reference. */
+int qh_hip_map_kspace(const qh_c64* fields, int count, int n, const qh_illumination* illum,
+                      const qh_odt_grid* grid, qh_c64* potential, uint16_t* counts,
+                      double* filled);
+
+/* ---- Pipeline (include/qoraholo/pipeline.hpp:50-85; src/backends.hpp:75-89,110-111) ---- */
+
+typedef struct qh_pipeline qh_pipeline;
+
+typedef struct {
+  int count;          /* holograms per stack */
+  int hologram_size;  /* side N of the square holograms */
+  qh_crop_plan crop;  /* resolved from FieldOptions and the carrier */
+  const qh_illumination* illuminations; /* count entries, copied */
+  qh_geometry geometry;
+  qh_odt_options odt;
+  int upload_chunks;
+} qh_pipeline_options;
+
+int qh_hip_pipeline_create(const qh_pipeline_options* options, qh_pipeline** out);
+void qh_hip_pipeline_destroy(qh_pipeline* p);
+/* Background holograms (count, N, N), host memory; fields are extracted once and kept. */
+int qh_hip_pipeline_set_background(qh_pipeline* p, const float* backgrounds);
+/* Page-locked input (count, N, N) and output (nz, n, n) buffers owned by the pipeline. */
+float* qh_hip_pipeline_input(qh_pipeline* p);
+const float* qh_hip_pipeline_output(const qh_pipeline* p);
+/* Reconstructs from input() into output(); QH_ERR_LOGIC without a background. */
+int qh_hip_pipeline_process(qh_pipeline* p);
+/* The same from holograms already in device memory into a device volume, on `stream`
+ * (a hipStream_t; NULL for the pipeline's own). Returns once the work is queued; the caller
+ * synchronises the stream before reading ri_device. */
+int qh_hip_pipeline_process_device(qh_pipeline* p, const float* holograms_device,
+                                   float* ri_device, void* stream);
+double qh_hip_pipeline_filled(const qh_pipeline* p);
+int qh_hip_pipeline_timings(const qh_pipeline* p, qh_pipeline_timings* out);
+double qh_hip_pipeline_setup_ms(const qh_pipeline* p);
+size_t qh_hip_pipeline_device_bytes(const qh_pipeline* p);
+int qh_hip_pipeline_output_shape(const qh_pipeline* p, int* nz, int* ny, int* nx);
+/* Records FGP kernel intervals in timings (a few events per outer iteration). */
+int qh_hip_pipeline_time_fgp(qh_pipeline* p, int on);
+
+/* ---- transform and TV test hooks (no counterpart in the reference's ABI: it used cuFFT) ---- */
+
+/* In-place complex FFT over the axes in `axes` (bit 0: z, 1: y, 2: x) of (d0, d1, d2). */
+int qh_hip_fft_c2c(qh_c64* data, int d0, int d1, int d2, unsigned axes, int forward, float scale);
+/* Real (nz, ny, nx) <-> half spectrum (nz, ny, nx/2+1). */
+int qh_hip_rfft3(const float* x, int nz, int ny, int nx, qh_c64* out);
+int qh_hip_irfft3(const qh_c64* in, int nz, int ny, int nx, float scale, float* x);
+/* One TV prox call from a zero dual: argmin 0.5 |x - b|^2 + lambda TV(x), x >= 0
+ * (src/cpu/tv.cpp:7-44). */
+int qh_hip_tv_prox(const float* b, int nz, int ny, int nx, float lambda, int inner, float* x);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QORAHOLO_HIP_H */

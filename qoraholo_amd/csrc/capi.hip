@@ -1,0 +1,511 @@
+// The C-ABI: argument checks with the reference's rules, status codes and per-thread error
+// messages, and the per-call entry points that copy host arrays in and out.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/qoraholo_hip.h"
+#include "fft.hpp"
+#include "pipeline.hpp"
+#include "runtime.hpp"
+#include "stages.hpp"
+
+struct qh_pipeline {
+  std::unique_ptr<qh::Pipeline> impl;
+};
+
+namespace qh {
+
+namespace {
+
+thread_local std::string g_error;
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    g_error.clear();
+    return QH_OK;
+  } catch (const InvalidArgument& e) {
+    g_error = e.what();
+    return QH_ERR_INVALID_ARGUMENT;
+  } catch (const LogicError& e) {
+    g_error = e.what();
+    return QH_ERR_LOGIC;
+  } catch (const OutOfMemory& e) {
+    g_error = e.what();
+    return QH_ERR_OUT_OF_MEMORY;
+  } catch (const std::bad_alloc&) {
+    g_error = "out of host memory";
+    return QH_ERR_OUT_OF_MEMORY;
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return QH_ERR_RUNTIME;
+  }
+}
+
+// Per-thread stream of the per-call entry points, so calls from different host threads run
+// independently.
+hipStream_t call_stream() {
+  thread_local std::unique_ptr<Stream> s;
+  if (!s) s = std::make_unique<Stream>();
+  return s->get();
+}
+
+void require_device() {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n < 1) throw DeviceError("no HIP device available");
+}
+
+void require_even_square(int ny, int nx, const char* who) {
+  require(ny == nx && nx >= 4 && nx % 2 == 0,
+          std::string(who) + ": images must be square with an even side");
+}
+
+void require_fft(int n, const char* who) {
+  require(fft_supported(n), std::string(who) + ": the HIP backend transforms sides of the form "
+                                               "2^a 3^b 5^c, got " + std::to_string(n));
+}
+
+OdtGrid make_grid(int n, int nz, const qh_geometry& g) {
+  const float fm = g.n_medium / g.wavelength;
+  const float fna = g.na / g.wavelength;
+  return {n, nz, 1.0f / (static_cast<float>(n) * g.pixel), 1.0f / (static_cast<float>(nz) * g.pixel),
+          fm * fm, fna * fna, 4.0f * kPi / g.pixel};
+}
+
+// src/api.cpp:107-123 (odt_grid): checks of reconstruct() and of the Pipeline.
+OdtGrid checked_grid(int n, int count, const qh_illumination* illum, const qh_geometry& g,
+                     const qh_odt_options& o, const char* who) {
+  require_even_square(n, n, who);
+  require(count >= 1 && illum != nullptr, std::string(who) + ": need one illumination per field");
+  require(g.wavelength > 0 && g.pixel > 0 && g.n_medium > 0 && g.na > 0,
+          std::string(who) + ": geometry values must be positive");
+  const int nz = o.nz > 0 ? o.nz : n;
+  require(nz % 2 == 0, std::string(who) + ": nz must be even");
+  require(o.iterations >= 0, std::string(who) + ": iterations must not be negative");
+  require(o.solver == 0 || o.solver == 1, std::string(who) + ": unknown solver");
+  require(o.gp_transform == 0 || o.gp_transform == 1, std::string(who) + ": unknown gp_transform");
+  require(o.solver != 1 || o.iterations == 0 || (o.tv_weight > 0 && o.tv_inner >= 0),
+          std::string(who) + ": TV needs tv_weight > 0");
+  const float fm = g.n_medium / g.wavelength;
+  for (int a = 0; a < count; ++a) {
+    require(illum[a].fy * illum[a].fy + illum[a].fx * illum[a].fx < fm * fm,
+            std::string(who) + ": illumination frequency beyond n_medium / wavelength");
+  }
+  require_fft(n, who);
+  require_fft(nz, who);
+  return make_grid(n, nz, g);
+}
+
+void check_plan(const qh_crop_plan& p, int n) {
+  require(p.out % 2 == 0 && p.out <= n && p.out >= 4,
+          "extract_fields: output_size must be even and <= N");
+  require(p.radius > 0 && p.radius <= static_cast<float>(p.out) / 2.0f,
+          "extract_fields: crop radius must fit in the output window");
+}
+
+}  // namespace
+
+OdtGrid grid_from(const qh_pipeline_options& o) {
+  return checked_grid(o.crop.out, o.count, o.illuminations, o.geometry, o.odt, "Pipeline");
+}
+
+OdtConfig config_from(const qh_odt_options& o, const qh_geometry& g) {
+  OdtConfig c;
+  c.iterations = o.iterations;
+  c.solver = o.solver;
+  c.gp_transform = o.gp_transform;
+  c.tv_weight = o.tv_weight;
+  c.tv_inner = o.tv_inner;
+  c.wavelength = g.wavelength;
+  c.n_medium = g.n_medium;
+  return c;
+}
+
+}  // namespace qh
+
+using namespace qh;
+
+extern "C" {
+
+const char* qh_hip_last_error(void) { return g_error.c_str(); }
+
+int qh_hip_device_present(void) {
+  int n = 0;
+  return hipGetDeviceCount(&n) == hipSuccess && n > 0 ? 1 : 0;
+}
+
+int qh_make_grid(int n, int nz, const qh_geometry* geometry, qh_odt_grid* out) {
+  return guarded([&] {
+    require(geometry && out, "make_grid: null argument");
+    const OdtGrid g = make_grid(n, nz, *geometry);
+    *out = {g.n, g.nz, g.df, g.dfz, g.fm2, g.fna2, g.gain};
+  });
+}
+
+int qh_resolve_crop(float center_y, float center_x, float radius, int output_size, int n,
+                 qh_crop_plan* out) {
+  return guarded([&] {
+    require(out != nullptr, "crop_plan: null argument");
+    qh_crop_plan p{static_cast<int>(std::lround(center_y)), static_cast<int>(std::lround(center_x)),
+                   radius, output_size > 0 ? output_size : n};
+    if (p.radius <= 0) p.radius = std::hypot(center_y, center_x) / 3.0f;
+    check_plan(p, n);
+    *out = p;
+  });
+}
+
+int qh_circular_scan(int count, float na_illumination, const qh_geometry* geometry,
+                     qh_illumination* out) {
+  return guarded([&] {
+    require(geometry && out, "circular_scan: null argument");
+    require(count >= 1, "circular_scan: count must be positive");
+    require(na_illumination >= 0 && na_illumination < geometry->n_medium,
+            "circular_scan: na_illumination must be below n_medium");
+    out[0] = {0.0f, 0.0f};
+    const float f = na_illumination / geometry->wavelength;
+    for (int i = 1; i < count; ++i) {
+      const float phi = 2.0f * kPi * static_cast<float>(i - 1) / static_cast<float>(count - 1);
+      out[i] = {f * std::sin(phi), f * std::cos(phi)};
+    }
+  });
+}
+
+int qh_gaussian_noise(uint64_t seed, float sigma, size_t count, float* out) {
+  return guarded([&] {
+    require(out != nullptr || count == 0, "gaussian_noise: null output");
+    std::mt19937_64 rng(seed);
+    std::normal_distribution<float> noise(0.0f, sigma);
+    for (size_t i = 0; i < count; ++i) out[i] = noise(rng);
+  });
+}
+
+int qh_hip_extract_fields(const float* holograms, int count, int n, const qh_crop_plan* plan,
+                          qh_c64* fields) {
+  return guarded([&] {
+    require(holograms && plan && fields, "extract_fields: null argument");
+    require_even_square(n, n, "extract_fields");
+    require(count >= 1, "extract_fields: empty stack");
+    check_plan(*plan, n);
+    require_fft(n, "extract_fields");
+    require_fft(plan->out, "extract_fields");
+    require_device();
+    const hipStream_t s = call_stream();
+    const int m = plan->out;
+    const std::size_t image = static_cast<std::size_t>(n) * n;
+    DeviceBuffer<float> holo(count * image);
+    DeviceBuffer<float2> half(static_cast<std::size_t>(count) * n * (n / 2 + 1));
+    DeviceBuffer<float2> out(static_cast<std::size_t>(count) * m * m);
+    check(hipMemcpyAsync(holo.get(), holograms, holo.bytes(), hipMemcpyHostToDevice, s), "upload");
+    hologram_spectra(holo.get(), count, n, half.get(), s);
+    const CropPlan cp{plan->cy, plan->cx, plan->radius, m};
+    crop_fields(half.get(), count, n, cp, out.get(), s);
+    fields_inverse(out.get(), count, m, 1.0f / static_cast<float>(image), true, s);
+    check(hipMemcpyAsync(fields, out.get(), out.bytes(), hipMemcpyDeviceToHost, s), "download");
+    check(hipStreamSynchronize(s), "extract_fields");
+  });
+}
+
+int qh_hip_unwrap(const float* wrapped, int nz, int ny, int nx, float* out) {
+  return guarded([&] {
+    require(wrapped && out, "unwrap: null argument");
+    require(nz >= 1 && ny % 2 == 0 && nx % 2 == 0 && ny >= 2 && nx >= 2,
+            "unwrap: image sides must be even");
+    require_fft(ny, "unwrap");
+    require_fft(nx, "unwrap");
+    require_device();
+    const hipStream_t s = call_stream();
+    const std::size_t total = static_cast<std::size_t>(nz) * ny * nx;
+    DeviceBuffer<float> psi(total);
+    DeviceBuffer<float> phi(total);
+    Unwrapper u(nz, ny, nx);
+    check(hipMemcpyAsync(psi.get(), wrapped, psi.bytes(), hipMemcpyHostToDevice, s), "upload");
+    u.unwrap(psi.get(), phi.get(), s);
+    check(hipMemcpyAsync(out, phi.get(), phi.bytes(), hipMemcpyDeviceToHost, s), "download");
+    check(hipStreamSynchronize(s), "unwrap");
+  });
+}
+
+int qh_hip_rytov(const qh_c64* ratio, int nz, int ny, int nx, qh_c64* out) {
+  return guarded([&] {
+    require(ratio && out, "rytov: null argument");
+    require(nz >= 1 && ny % 2 == 0 && nx % 2 == 0 && ny >= 4 && nx >= 4,
+            "rytov: image sides must be even");
+    require_fft(ny, "rytov");
+    require_fft(nx, "rytov");
+    require_device();
+    const hipStream_t s = call_stream();
+    const std::size_t total = static_cast<std::size_t>(nz) * ny * nx;
+    DeviceBuffer<float2> r(total);
+    DeviceBuffer<float> psi(total);
+    DeviceBuffer<float> logamp(total);
+    Unwrapper u(nz, ny, nx);
+    check(hipMemcpyAsync(r.get(), ratio, r.bytes(), hipMemcpyHostToDevice, s), "upload");
+    split_ratio(r.get(), static_cast<long>(total), psi.get(), logamp.get(), s);
+    u.rytov(psi.get(), logamp.get(), r.get(), s);
+    check(hipMemcpyAsync(out, r.get(), r.bytes(), hipMemcpyDeviceToHost, s), "download");
+    check(hipStreamSynchronize(s), "rytov");
+  });
+}
+
+int qh_hip_reconstruct(const qh_c64* fields, int count, int n, const qh_illumination* illum,
+                       const qh_odt_grid* grid, const qh_geometry* geometry,
+                       const qh_odt_options* options, float* ri, double* filled,
+                       qh_odt_timings* timings) {
+  return guarded([&] {
+    require(fields && illum && geometry && options && ri, "reconstruct: null argument");
+    const OdtGrid checked = checked_grid(n, count, illum, *geometry, *options, "reconstruct");
+    OdtGrid g = checked;
+    if (grid) g = {grid->n, grid->nz, grid->df, grid->dfz, grid->fm2, grid->fna2, grid->gain};
+    require(g.n == n && g.nz == checked.nz, "reconstruct: grid does not match the fields");
+    require_device();
+    const auto t0 = std::chrono::steady_clock::now();
+    std::size_t free_before = 0, free_after = 0, total_mem = 0;
+    check(hipMemGetInfo(&free_before, &total_mem), "hipMemGetInfo");
+    const hipStream_t s = call_stream();
+    OdtSolver solver(g, count, config_from(*options, *geometry));
+    std::vector<float> il(2 * static_cast<std::size_t>(count));
+    for (int a = 0; a < count; ++a) {
+      il[2 * a] = illum[a].fy;
+      il[2 * a + 1] = illum[a].fx;
+    }
+    solver.set_illuminations(il.data());
+    solver.time_fgp(timings != nullptr);
+    const std::size_t plane = static_cast<std::size_t>(n) * n;
+    DeviceBuffer<float2> f(plane * count);
+    for (int len : {n, n / 2, g.nz}) fft_twiddles(len);
+    check(hipMemGetInfo(&free_after, &total_mem), "hipMemGetInfo");
+    const double setup =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    StageClock clock(kOdtMarks + 3);
+    const int start = kOdtMarks, uploaded = kOdtMarks + 1, downloaded = kOdtMarks + 2;
+    clock.mark(start, s);
+    check(hipMemcpyAsync(f.get(), fields, f.bytes(), hipMemcpyHostToDevice, s), "upload");
+    clock.mark(uploaded, s);
+    solver.run(f.get(), s, &clock, 0);
+    const std::size_t vol = plane * g.nz;
+    unsigned long long nfilled = 0;
+    check(hipMemcpyAsync(ri, solver.index(), vol * sizeof(float), hipMemcpyDeviceToHost, s), "download");
+    check(hipMemcpyAsync(&nfilled, solver.filled_device(), sizeof nfilled, hipMemcpyDeviceToHost, s),
+          "download");
+    clock.mark(downloaded, s);
+    check(hipStreamSynchronize(s), "reconstruct");
+    if (filled) *filled = static_cast<double>(nfilled) / static_cast<double>(vol);
+    if (timings) {
+      timings->bytes = free_before > free_after ? free_before - free_after : 0;
+      timings->setup = setup;
+      timings->upload = clock.ms(start, uploaded);
+      timings->fft2 = clock.ms(uploaded, kFft2);
+      timings->clear = clock.ms(kFft2, kClear);
+      timings->mapping = clock.ms(kClear, kMapping);
+      timings->normalize = clock.ms(kMapping, kNormalize);
+      timings->solve = clock.ms(kNormalize, kSolve);
+      timings->finish = clock.ms(kSolve, kFinish);
+      timings->download = clock.ms(kFinish, downloaded);
+      timings->fgp_ms = solver.collect_fgp_ms();
+      timings->fgp_steps = solver.fgp_steps();
+    }
+  });
+}
+
+int qh_hip_map_kspace(const qh_c64* fields, int count, int n, const qh_illumination* illum,
+                      const qh_odt_grid* grid, qh_c64* potential, uint16_t* counts,
+                      double* filled) {
+  return guarded([&] {
+    require(fields && illum && grid && potential && counts, "map_kspace: null argument");
+    require_even_square(n, n, "map_kspace");
+    require(grid->n == n && grid->nz % 2 == 0 && grid->nz >= 2, "map_kspace: bad grid");
+    require(count >= 1, "map_kspace: need one illumination per field");
+    require_fft(n, "map_kspace");
+    require_device();
+    const hipStream_t s = call_stream();
+    const OdtGrid g{grid->n, grid->nz, grid->df, grid->dfz, grid->fm2, grid->fna2, grid->gain};
+    OdtSolver solver(g, count, OdtConfig{});
+    std::vector<float> il(2 * static_cast<std::size_t>(count));
+    for (int a = 0; a < count; ++a) {
+      il[2 * a] = illum[a].fy;
+      il[2 * a + 1] = illum[a].fx;
+    }
+    solver.set_illuminations(il.data());
+    const std::size_t plane = static_cast<std::size_t>(n) * n;
+    DeviceBuffer<float2> f(plane * count);
+    check(hipMemcpyAsync(f.get(), fields, f.bytes(), hipMemcpyHostToDevice, s), "upload");
+    solver.map_only(f.get(), s);
+    const std::size_t vol = plane * g.nz;
+    unsigned long long nfilled = 0;
+    check(hipMemcpyAsync(potential, solver.potential(), vol * sizeof(float2), hipMemcpyDeviceToHost, s),
+          "download");
+    check(hipMemcpyAsync(counts, solver.counts(), vol * sizeof(uint16_t), hipMemcpyDeviceToHost, s),
+          "download");
+    check(hipMemcpyAsync(&nfilled, solver.filled_device(), sizeof nfilled, hipMemcpyDeviceToHost, s),
+          "download");
+    check(hipStreamSynchronize(s), "map_kspace");
+    if (filled) *filled = static_cast<double>(nfilled) / static_cast<double>(vol);
+  });
+}
+
+int qh_hip_pipeline_create(const qh_pipeline_options* o, qh_pipeline** out) {
+  return guarded([&] {
+    require(o && out, "Pipeline: null argument");
+    *out = nullptr;
+    const int n = o->hologram_size;
+    require(o->count >= 1, "Pipeline: count must be positive");
+    require(n >= 4 && n % 2 == 0, "Pipeline: hologram_size must be even");
+    require(o->upload_chunks >= 1 && o->upload_chunks <= o->count,
+            "Pipeline: upload_chunks must be between 1 and count");
+    check_plan(o->crop, n);
+    require_fft(n, "Pipeline");
+    require_fft(o->crop.out, "Pipeline");
+    checked_grid(o->crop.out, o->count, o->illuminations, o->geometry, o->odt, "Pipeline");
+    require_device();
+    const auto t0 = std::chrono::steady_clock::now();
+    std::size_t free_before = 0, free_after = 0, total_mem = 0;
+    check(hipMemGetInfo(&free_before, &total_mem), "hipMemGetInfo");
+    auto p = std::make_unique<qh_pipeline>();
+    p->impl = std::make_unique<Pipeline>(*o);
+    check(hipDeviceSynchronize(), "Pipeline");
+    check(hipMemGetInfo(&free_after, &total_mem), "hipMemGetInfo");
+    p->impl->set_setup(
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(),
+        free_before > free_after ? free_before - free_after : 0);
+    *out = p.release();
+  });
+}
+
+void qh_hip_pipeline_destroy(qh_pipeline* p) { delete p; }
+
+int qh_hip_pipeline_set_background(qh_pipeline* p, const float* backgrounds) {
+  return guarded([&] {
+    require(p && backgrounds, "Pipeline::set_background: null argument");
+    p->impl->set_background(backgrounds);
+  });
+}
+
+float* qh_hip_pipeline_input(qh_pipeline* p) { return p ? p->impl->input() : nullptr; }
+
+const float* qh_hip_pipeline_output(const qh_pipeline* p) { return p ? p->impl->output() : nullptr; }
+
+int qh_hip_pipeline_process(qh_pipeline* p) {
+  return guarded([&] {
+    require(p != nullptr, "Pipeline: null handle");
+    p->impl->process();
+  });
+}
+
+int qh_hip_pipeline_process_device(qh_pipeline* p, const float* holograms_device, float* ri_device,
+                                   void* stream) {
+  return guarded([&] {
+    require(p && holograms_device && ri_device, "Pipeline: null argument");
+    p->impl->process_device(holograms_device, ri_device, static_cast<hipStream_t>(stream));
+  });
+}
+
+double qh_hip_pipeline_filled(const qh_pipeline* p) { return p ? p->impl->filled() : 0.0; }
+
+int qh_hip_pipeline_timings(const qh_pipeline* p, qh_pipeline_timings* out) {
+  return guarded([&] {
+    require(p && out, "Pipeline: null argument");
+    *out = const_cast<qh_pipeline*>(p)->impl->timings();
+  });
+}
+
+double qh_hip_pipeline_setup_ms(const qh_pipeline* p) { return p ? p->impl->setup_ms() : 0.0; }
+
+size_t qh_hip_pipeline_device_bytes(const qh_pipeline* p) { return p ? p->impl->device_bytes() : 0; }
+
+int qh_hip_pipeline_output_shape(const qh_pipeline* p, int* nz, int* ny, int* nx) {
+  return guarded([&] {
+    require(p && nz && ny && nx, "Pipeline: null argument");
+    *nz = p->impl->nz();
+    *ny = p->impl->n();
+    *nx = p->impl->n();
+  });
+}
+
+int qh_hip_pipeline_time_fgp(qh_pipeline* p, int on) {
+  return guarded([&] {
+    require(p != nullptr, "Pipeline: null handle");
+    p->impl->time_fgp(on != 0);
+  });
+}
+
+int qh_hip_fft_c2c(qh_c64* data, int d0, int d1, int d2, unsigned axes, int forward, float scale) {
+  return guarded([&] {
+    require(data && d0 >= 1 && d1 >= 1 && d2 >= 1, "fft: bad arguments");
+    if (axes & 1u) require_fft(d0, "fft");
+    if (axes & 2u) require_fft(d1, "fft");
+    if (axes & 4u) require_fft(d2, "fft");
+    require_device();
+    const hipStream_t s = call_stream();
+    DeviceBuffer<float2> d(static_cast<std::size_t>(d0) * d1 * d2);
+    check(hipMemcpyAsync(d.get(), data, d.bytes(), hipMemcpyHostToDevice, s), "upload");
+    fft3(d.get(), d0, d1, d2, axes, forward == 0, scale, s);
+    check(hipMemcpyAsync(data, d.get(), d.bytes(), hipMemcpyDeviceToHost, s), "download");
+    check(hipStreamSynchronize(s), "fft");
+  });
+}
+
+int qh_hip_rfft3(const float* x, int nz, int ny, int nx, qh_c64* out) {
+  return guarded([&] {
+    require(x && out && nz >= 1 && ny >= 1 && nx >= 2 && nx % 2 == 0, "rfft3: bad arguments");
+    require_fft(nz, "rfft3");
+    require_fft(ny, "rfft3");
+    require_fft(nx / 2, "rfft3");
+    require_device();
+    const hipStream_t s = call_stream();
+    const std::size_t vol = static_cast<std::size_t>(nz) * ny * nx;
+    const std::size_t half = static_cast<std::size_t>(nz) * ny * (nx / 2 + 1);
+    DeviceBuffer<float> dx(vol);
+    DeviceBuffer<float2> dh(half);
+    check(hipMemcpyAsync(dx.get(), x, dx.bytes(), hipMemcpyHostToDevice, s), "upload");
+    rfft3(dx.get(), dh.get(), nz, ny, nx, s);
+    check(hipMemcpyAsync(out, dh.get(), dh.bytes(), hipMemcpyDeviceToHost, s), "download");
+    check(hipStreamSynchronize(s), "rfft3");
+  });
+}
+
+int qh_hip_irfft3(const qh_c64* in, int nz, int ny, int nx, float scale, float* x) {
+  return guarded([&] {
+    require(x && in && nz >= 1 && ny >= 1 && nx >= 2 && nx % 2 == 0, "irfft3: bad arguments");
+    require_fft(nz, "irfft3");
+    require_fft(ny, "irfft3");
+    require_fft(nx / 2, "irfft3");
+    require_device();
+    const hipStream_t s = call_stream();
+    const std::size_t vol = static_cast<std::size_t>(nz) * ny * nx;
+    const std::size_t half = static_cast<std::size_t>(nz) * ny * (nx / 2 + 1);
+    DeviceBuffer<float> dx(vol);
+    DeviceBuffer<float2> dh(half);
+    check(hipMemcpyAsync(dh.get(), in, dh.bytes(), hipMemcpyHostToDevice, s), "upload");
+    irfft3(dh.get(), dx.get(), nz, ny, nx, scale, s);
+    check(hipMemcpyAsync(x, dx.get(), dx.bytes(), hipMemcpyDeviceToHost, s), "download");
+    check(hipStreamSynchronize(s), "irfft3");
+  });
+}
+
+int qh_hip_tv_prox(const float* b, int nz, int ny, int nx, float lambda, int inner, float* x) {
+  return guarded([&] {
+    require(b && x && nz >= 1 && ny >= 1 && nx >= 1 && lambda > 0 && inner >= 0,
+            "tv_prox: bad arguments");
+    require_device();
+    const hipStream_t s = call_stream();
+    const std::size_t vol = static_cast<std::size_t>(nz) * ny * nx;
+    DeviceBuffer<float> db(vol);
+    DeviceBuffer<float> dx(vol);
+    TvState state(nz, ny, nx);
+    state.reset(s);
+    check(hipMemcpyAsync(db.get(), b, db.bytes(), hipMemcpyHostToDevice, s), "upload");
+    state.prox(db.get(), dx.get(), lambda, inner, s);
+    check(hipMemcpyAsync(x, dx.get(), dx.bytes(), hipMemcpyDeviceToHost, s), "download");
+    check(hipStreamSynchronize(s), "tv_prox");
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,414 @@
+// Batched FFT kernels on LDS-resident lines and their host launchers.
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "common.hpp"
+#include "fft.hpp"
+#include "runtime.hpp"
+
+namespace qh {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+std::mutex g_tables_mu;
+// (device, kind, n) -> table; kind 0: FFT twiddles, 1: DCT twiddles.
+std::map<std::tuple<int, int, int>, DeviceBuffer<float2>> g_tables;
+
+const float2* table(int kind, int n) {
+  int dev = 0;
+  check(hipGetDevice(&dev), "hipGetDevice");
+  std::lock_guard<std::mutex> lock(g_tables_mu);
+  auto& buf = g_tables[{dev, kind, n}];
+  if (buf.size() == 0) {
+    std::vector<float2> h(n);
+    for (int t = 0; t < n; ++t) {
+      // kind 0: exp(-2 pi i t / n); kind 1: exp(-i pi t / (2 n)). Evaluated in double.
+      const double a = kind == 0 ? 2.0 * M_PI * t / static_cast<double>(n)
+                                 : M_PI * t / (2.0 * static_cast<double>(n));
+      h[t] = {static_cast<float>(std::cos(a)), static_cast<float>(-std::sin(a))};
+    }
+    DeviceBuffer<float2> d(n);
+    check(hipMemcpy(d.get(), h.data(), n * sizeof(float2), hipMemcpyHostToDevice), "upload twiddles");
+    buf = std::move(d);
+  }
+  return buf.get();
+}
+
+__device__ __forceinline__ long line_base(long l, const Lines& g) {
+  return (l / g.inner) * g.outer + (l % g.inner) * g.inner_stride;
+}
+
+struct LinesArgs {
+  const float2* in;
+  float2* out;
+  Lines g;
+  int n;
+  int lines_per_block;
+  FftFactors f;
+  const float2* tw;
+  int inverse;
+  float scale;
+  int alt_mod;
+};
+
+// CONTIG: lines contiguous in memory (stride 1), LDS line-major, threads walk along lines.
+// Otherwise lines are adjacent (inner_stride 1): each row of the block is one coalesced
+// segment, LDS is interleaved (line fastest) and threads walk across lines.
+template <bool CONTIG>
+__global__ void __launch_bounds__(kThreads) k_fft_lines(LinesArgs a) {
+  extern __shared__ float2 smem[];
+  __shared__ long base[64];
+  const int n = a.n;
+  float2* tw = smem;
+  float2* b0 = smem + n;
+  float2* b1 = b0 + a.lines_per_block * n;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = a.tw[i];
+  const long l0 = static_cast<long>(blockIdx.x) * a.lines_per_block;
+  const int nl = static_cast<int>(min(static_cast<long>(a.lines_per_block), a.g.count - l0));
+  for (int i = threadIdx.x; i < nl; i += blockDim.x) base[i] = line_base(l0 + i, a.g);
+  __syncthreads();
+  const int ls = CONTIG ? n : 1;
+  const int es = CONTIG ? 1 : nl;
+  const int total = nl * n;
+  for (int e = threadIdx.x; e < total; e += blockDim.x) {
+    int line, i;
+    if (CONTIG) {
+      line = e / n;
+      i = e - line * n;
+    } else {
+      i = e / nl;
+      line = e - i * nl;
+    }
+    b0[line * ls + i * es] = a.in[base[line] + i * a.g.stride];
+  }
+  __syncthreads();
+  const float2* r = lds_fft(b0, b1, ls, es, nl, a.f, tw, a.inverse != 0, !CONTIG);
+  for (int e = threadIdx.x; e < total; e += blockDim.x) {
+    int line, i;
+    if (CONTIG) {
+      line = e / n;
+      i = e - line * n;
+    } else {
+      i = e / nl;
+      line = e - i * nl;
+    }
+    float2 v = r[line * ls + i * es];
+    if (a.scale != 1.0f) v = cscale(v, a.scale);
+    if (a.alt_mod > 0 && ((i + (l0 + line) % a.alt_mod) & 1)) v = {-v.x, -v.y};
+    a.out[base[line] + i * a.g.stride] = v;
+  }
+}
+
+struct RowsArgs {
+  const float* x;
+  const float* x2;
+  float* xo;
+  long x_pitch;
+  float2* c;
+  long c_pitch;
+  long rows;
+  int n;
+  int rows_per_block;
+  FftFactors f;      // of n / 2
+  const float2* tw;  // exp(-2 pi i t / n), t in [0, n)
+  RealOp op;
+};
+
+// Real rows of even length n through a complex transform of n/2 packed samples
+// z[j] = x[2j] + i x[2j+1]; X[k] = (Z[k] + conj Z[-k]) / 2 - i/2 w^k (Z[k] - conj Z[-k]).
+__global__ void __launch_bounds__(kThreads) k_rfft_rows(RowsArgs a) {
+  extern __shared__ float2 smem[];
+  const int n = a.n;
+  const int m = n / 2;
+  float2* tw = smem;        // n entries
+  float2* twm = tw + n;     // m entries, exp(-2 pi i t / m)
+  float2* b0 = twm + m;
+  float2* b1 = b0 + a.rows_per_block * (m + 1);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const float2 w = a.tw[i];
+    tw[i] = w;
+    if ((i & 1) == 0) twm[i >> 1] = w;
+  }
+  const long r0 = static_cast<long>(blockIdx.x) * a.rows_per_block;
+  const int nl = static_cast<int>(min(static_cast<long>(a.rows_per_block), a.rows - r0));
+  for (int e = threadIdx.x; e < nl * m; e += blockDim.x) {
+    const int line = e / m;
+    const int j = e - line * m;
+    const long at = (r0 + line) * a.x_pitch + 2 * j;
+    float u = a.x[at];
+    float v = a.x[at + 1];
+    if (a.op.in == RealIn::nonneg) {
+      u = u < 0.0f ? 0.0f : u;
+      v = v < 0.0f ? 0.0f : v;
+    } else if (a.op.in == RealIn::momentum) {
+      const float pu = a.x2[at];
+      const float pv = a.x2[at + 1];
+      u = u + a.op.a * (u - pu);
+      v = v + a.op.a * (v - pv);
+    }
+    b0[line * m + j] = {u, v};
+  }
+  __syncthreads();
+  const float2* z = lds_fft(b0, b1, m, 1, nl, a.f, twm, false, false);
+  for (int e = threadIdx.x; e < nl * (m + 1); e += blockDim.x) {
+    const int line = e / (m + 1);
+    const int k = e - line * (m + 1);
+    const float2 zk = z[line * m + (k == m ? 0 : k)];
+    const float2 zm = cconj(z[line * m + (k == 0 ? 0 : m - k)]);
+    const float2 ev = {0.5f * (zk.x + zm.x), 0.5f * (zk.y + zm.y)};
+    const float2 d = {0.5f * (zk.x - zm.x), 0.5f * (zk.y - zm.y)};
+    const float2 od = cmul(tw[k], d);
+    const float2 o = {od.y, -od.x};
+    float2 out;
+    if (k == 0 || k == m) {
+      out = {ev.x + o.x, 0.0f};
+    } else {
+      out = cadd(ev, o);
+    }
+    a.c[(r0 + line) * a.c_pitch + k] = out;
+  }
+}
+
+// Inverse: Z[k] = (X[k] + conj X[m-k]) + i w^-k (X[k] - conj X[m-k]), z = IFFT_m(Z),
+// x[2j] = Re z[j], x[2j+1] = Im z[j]; imaginary parts of X[0] and X[m] are dropped.
+__global__ void __launch_bounds__(kThreads) k_irfft_rows(RowsArgs a) {
+  extern __shared__ float2 smem[];
+  const int n = a.n;
+  const int m = n / 2;
+  float2* tw = smem;
+  float2* twm = tw + n;
+  float2* b0 = twm + m;
+  float2* b1 = b0 + a.rows_per_block * (m + 1);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const float2 w = a.tw[i];
+    tw[i] = w;
+    if ((i & 1) == 0) twm[i >> 1] = w;
+  }
+  const long r0 = static_cast<long>(blockIdx.x) * a.rows_per_block;
+  const int nl = static_cast<int>(min(static_cast<long>(a.rows_per_block), a.rows - r0));
+  for (int e = threadIdx.x; e < nl * (m + 1); e += blockDim.x) {
+    const int line = e / (m + 1);
+    const int k = e - line * (m + 1);
+    b0[e] = a.c[(r0 + line) * a.c_pitch + k];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < nl * m; e += blockDim.x) {
+    const int line = e / m;
+    const int k = e - line * m;
+    float2 xk = b0[line * (m + 1) + k];
+    float2 xm = b0[line * (m + 1) + m - k];
+    if (k == 0) {
+      xk.y = 0.0f;
+      xm.y = 0.0f;
+    }
+    const float2 s = cadd(xk, cconj(xm));
+    const float2 d = csub(xk, cconj(xm));
+    const float2 wd = cmul(cconj(tw[k]), d);
+    b1[line * m + k] = cadd(s, float2{-wd.y, wd.x});
+  }
+  __syncthreads();
+  const float2* z = lds_fft(b1, b0, m, 1, nl, a.f, twm, true, false);
+  for (int e = threadIdx.x; e < nl * m; e += blockDim.x) {
+    const int line = e / m;
+    const int j = e - line * m;
+    const float2 v = z[line * m + j];
+    float u0 = v.x * a.op.a;
+    float u1 = v.y * a.op.a;
+    if (a.op.out == RealOut::index) {
+      u0 = index_from_potential(u0, 0.0f, a.op.nm2, a.op.inv_k02);
+      u1 = index_from_potential(u1, 0.0f, a.op.nm2, a.op.inv_k02);
+    }
+    float* px = a.xo + (r0 + line) * a.x_pitch + 2 * j;
+    px[0] = u0;
+    px[1] = u1;
+  }
+}
+
+struct ZArgs {
+  float2* e;
+  const float2* a;
+  const std::uint8_t* hits;
+  long lines;
+  long plane;
+  int nz;
+  int lines_per_block;
+  FftFactors f;
+  const float2* tw;
+};
+
+__global__ void __launch_bounds__(kThreads) k_z_roundtrip_restore(ZArgs a) {
+  extern __shared__ float2 smem[];
+  const int n = a.nz;
+  float2* tw = smem;
+  float2* b0 = smem + n;
+  float2* b1 = b0 + a.lines_per_block * n;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = a.tw[i];
+  const long l0 = static_cast<long>(blockIdx.x) * a.lines_per_block;
+  const int nl = static_cast<int>(min(static_cast<long>(a.lines_per_block), a.lines - l0));
+  const int total = nl * n;
+  for (int e = threadIdx.x; e < total; e += blockDim.x) {
+    const int i = e / nl;
+    const int line = e - i * nl;
+    b0[e] = a.e[i * a.plane + l0 + line];
+  }
+  __syncthreads();
+  float2* r = lds_fft(b0, b1, 1, nl, nl, a.f, tw, false, true);
+  for (int e = threadIdx.x; e < total; e += blockDim.x) {
+    const int i = e / nl;
+    const int line = e - i * nl;
+    const long g = i * a.plane + l0 + line;
+    const float2 d = a.a[g];
+    float2 v = r[e];
+    hermitian_restore(d.x, d.y, a.hits[g], v.x, v.y);
+    r[e] = v;
+  }
+  __syncthreads();
+  const float2* r2 = lds_fft(r, r == b0 ? b1 : b0, 1, nl, nl, a.f, tw, true, true);
+  for (int e = threadIdx.x; e < total; e += blockDim.x) {
+    const int i = e / nl;
+    const int line = e - i * nl;
+    a.e[i * a.plane + l0 + line] = r2[e];
+  }
+}
+
+int contig_lines_per_block(int n) { return std::max(1, std::min(64, 2048 / n)); }
+int strided_lines_per_block(int n) { return n <= 256 ? 16 : std::max(1, std::min(16, 4096 / n)); }
+
+}  // namespace
+
+FftFactors fft_factors(int n) {
+  require(n >= 1, "fft: length must be positive");
+  FftFactors f;
+  f.n = n;
+  int m = n;
+  const auto push = [&](int r) {
+    require(f.count < kMaxFactors, "fft: too many factors");
+    f.r[f.count++] = r;
+    m /= r;
+  };
+  while (m % 8 == 0) push(8);
+  while (m % 4 == 0) push(4);
+  while (m % 2 == 0) push(2);
+  while (m % 3 == 0) push(3);
+  while (m % 5 == 0) push(5);
+  require(m == 1, "fft: lengths must factor into 2, 3 and 5, got " + std::to_string(n));
+  return f;
+}
+
+bool fft_supported(int n) {
+  if (n < 1) return false;
+  for (int r : {2, 3, 5}) {
+    while (n % r == 0) n /= r;
+  }
+  return n == 1;
+}
+
+const float2* fft_twiddles(int n) { return table(0, n); }
+const float2* dct_twiddles(int n) { return table(1, n); }
+
+Lines lines_along(int axis, int d0, int d1, int d2, long pitch) {
+  switch (axis) {
+    case 2: return {static_cast<long>(d0) * d1, d1, d1 * pitch, pitch, 1};
+    case 1: return {static_cast<long>(d0) * d2, d2, d1 * pitch, 1, pitch};
+    default: return {static_cast<long>(d1) * d2, d2, pitch, 1, d1 * pitch};
+  }
+}
+
+void fft_lines(const float2* in, float2* out, int n, const Lines& l, bool inverse, float scale,
+               int alt_mod, hipStream_t s) {
+  if (l.count == 0) return;
+  LinesArgs a{in, out, l, n, 0, fft_factors(n), fft_twiddles(n), inverse ? 1 : 0, scale, alt_mod};
+  if (l.stride == 1) {
+    a.lines_per_block = contig_lines_per_block(n);
+    const std::size_t lds = (n + 2 * static_cast<std::size_t>(a.lines_per_block) * n) * sizeof(float2);
+    const unsigned grid = static_cast<unsigned>((l.count + a.lines_per_block - 1) / a.lines_per_block);
+    hipLaunchKernelGGL(k_fft_lines<true>, dim3(grid), dim3(kThreads), lds, s, a);
+  } else {
+    require(l.inner_stride == 1, "fft: strided lines must be adjacent");
+    a.lines_per_block = strided_lines_per_block(n);
+    const std::size_t lds = (n + 2 * static_cast<std::size_t>(a.lines_per_block) * n) * sizeof(float2);
+    const unsigned grid = static_cast<unsigned>((l.count + a.lines_per_block - 1) / a.lines_per_block);
+    hipLaunchKernelGGL(k_fft_lines<false>, dim3(grid), dim3(kThreads), lds, s, a);
+  }
+  check_launch("fft_lines");
+}
+
+void rfft_rows(const float* x, const float* x2, long x_pitch, float2* out, long out_pitch,
+               long rows, int n, const RealOp& op, hipStream_t s) {
+  require(n % 2 == 0, "rfft: even length required");
+  if (rows == 0) return;
+  const int m = n / 2;
+  RowsArgs a{x, x2, nullptr, x_pitch, out, out_pitch, rows, n, contig_lines_per_block(m),
+             fft_factors(m), fft_twiddles(n), op};
+  const std::size_t lds =
+      (n + m + static_cast<std::size_t>(a.rows_per_block) * (2 * m + 1)) * sizeof(float2);
+  const unsigned grid = static_cast<unsigned>((rows + a.rows_per_block - 1) / a.rows_per_block);
+  hipLaunchKernelGGL(k_rfft_rows, dim3(grid), dim3(kThreads), lds, s, a);
+  check_launch("rfft_rows");
+}
+
+void irfft_rows(const float2* in, long in_pitch, float* x, long x_pitch, long rows, int n,
+                const RealOp& op, hipStream_t s) {
+  require(n % 2 == 0, "irfft: even length required");
+  if (rows == 0) return;
+  const int m = n / 2;
+  RowsArgs a{nullptr, nullptr, x, x_pitch, const_cast<float2*>(in), in_pitch, rows, n,
+             contig_lines_per_block(m), fft_factors(m), fft_twiddles(n), op};
+  const std::size_t lds =
+      (n + m + static_cast<std::size_t>(a.rows_per_block) * (2 * m + 1)) * sizeof(float2);
+  const unsigned grid = static_cast<unsigned>((rows + a.rows_per_block - 1) / a.rows_per_block);
+  hipLaunchKernelGGL(k_irfft_rows, dim3(grid), dim3(kThreads), lds, s, a);
+  check_launch("irfft_rows");
+}
+
+void z_roundtrip_restore(float2* e, const float2* a, const std::uint8_t* hits, int nz, int ny,
+                         int h, hipStream_t s) {
+  const long plane = static_cast<long>(ny) * h;
+  ZArgs z{e, a, hits, plane, plane, nz, strided_lines_per_block(nz), fft_factors(nz),
+          fft_twiddles(nz)};
+  const std::size_t lds = (nz + 2 * static_cast<std::size_t>(z.lines_per_block) * nz) * sizeof(float2);
+  const unsigned grid = static_cast<unsigned>((plane + z.lines_per_block - 1) / z.lines_per_block);
+  hipLaunchKernelGGL(k_z_roundtrip_restore, dim3(grid), dim3(kThreads), lds, s, z);
+  check_launch("z_roundtrip_restore");
+}
+
+void rfft3(const float* x, float2* out, int nz, int ny, int nx, hipStream_t s) {
+  const int h = nx / 2 + 1;
+  rfft_rows(x, nullptr, nx, out, h, static_cast<long>(nz) * ny, nx, RealOp{}, s);
+  fft_lines(out, out, ny, lines_along(1, nz, ny, h, h), false, 1.0f, 0, s);
+  fft_lines(out, out, nz, lines_along(0, nz, ny, h, h), false, 1.0f, 0, s);
+}
+
+void irfft3(float2* in, float* x, int nz, int ny, int nx, float scale, hipStream_t s) {
+  const int h = nx / 2 + 1;
+  fft_lines(in, in, nz, lines_along(0, nz, ny, h, h), true, 1.0f, 0, s);
+  fft_lines(in, in, ny, lines_along(1, nz, ny, h, h), true, 1.0f, 0, s);
+  RealOp op;
+  op.a = scale;
+  irfft_rows(in, h, x, nx, static_cast<long>(nz) * ny, nx, op, s);
+}
+
+void fft3(float2* data, int nz, int ny, int nx, unsigned axes, bool inverse, float scale,
+          hipStream_t s) {
+  const int dims[3] = {nz, ny, nx};
+  int first = 3;
+  for (int ax = 0; ax < 3; ++ax) {
+    if (axes & (1u << ax)) first = std::min(first, ax);
+  }
+  // Passes run x, y, z; the scale goes with the last one.
+  for (int ax = 2; ax >= 0; --ax) {
+    if (!(axes & (1u << ax))) continue;
+    const bool final_pass = ax == first;
+    fft_lines(data, data, dims[ax], lines_along(ax, nz, ny, nx, nx), inverse,
+              final_pass ? scale : 1.0f, 0, s);
+  }
+}
+
+}  // namespace qh

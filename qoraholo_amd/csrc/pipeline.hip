@@ -1,0 +1,173 @@
+// Device-resident pipeline. The sample holograms are copied in `upload_chunks` parts on a copy
+// stream while the compute stream transforms and crops the parts already present; fields stay
+// in HBM from the crop to the index volume, which leaves with one copy.
+#include <algorithm>
+#include <chrono>
+
+#include "fft.hpp"
+#include "pipeline.hpp"
+
+namespace qh {
+
+OdtGrid grid_from(const qh_pipeline_options& o);
+OdtConfig config_from(const qh_odt_options& o, const qh_geometry& g);
+
+namespace {
+
+double since_ms(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace
+
+Pipeline::Pipeline(const qh_pipeline_options& o)
+    : count_(o.count),
+      n_(o.hologram_size),
+      m_(o.crop.out),
+      chunk_((o.count + o.upload_chunks - 1) / o.upload_chunks),
+      chunks_((o.count + chunk_ - 1) / chunk_),
+      crop_{o.crop.cy, o.crop.cx, o.crop.radius, o.crop.out},
+      unwrapper_(o.count, o.crop.out, o.crop.out),
+      solver_(grid_from(o), o.count, config_from(o.odt, o.geometry)),
+      holo_(static_cast<std::size_t>(o.count) * o.hologram_size * o.hologram_size),
+      half_(static_cast<std::size_t>(o.count) * o.hologram_size * (o.hologram_size / 2 + 1)),
+      background_(static_cast<std::size_t>(o.count) * o.crop.out * o.crop.out),
+      fields_(background_.size()),
+      psi_(background_.size()),
+      logamp_(background_.size()),
+      input_(holo_.size()),
+      output_(static_cast<std::size_t>(solver_.nz()) * solver_.n() * solver_.n()),
+      filled_(1),
+      chunk_done_(chunks_),
+      clock_(kMarks) {
+  std::vector<float> il(2 * static_cast<std::size_t>(count_));
+  for (int a = 0; a < count_; ++a) {
+    il[2 * a] = o.illuminations[a].fy;
+    il[2 * a + 1] = o.illuminations[a].fx;
+  }
+  solver_.set_illuminations(il.data());
+  // Twiddle tables of every transform length, created now so process() allocates nothing.
+  for (int len : {n_, n_ / 2, m_, solver_.n(), solver_.n() / 2, solver_.nz()}) {
+    if (len >= 1) {
+      fft_twiddles(len);
+      dct_twiddles(len);
+    }
+  }
+  *filled_.get() = 0;
+}
+
+void Pipeline::set_background(const float* host) {
+  const hipStream_t s = compute_.get();
+  const std::size_t image = static_cast<std::size_t>(n_) * n_;
+  check(hipMemcpyAsync(holo_.get(), host, count_ * image * sizeof(float), hipMemcpyHostToDevice, s),
+        "upload backgrounds");
+  hologram_spectra(holo_.get(), count_, n_, half_.get(), s);
+  crop_fields(half_.get(), count_, n_, crop_, background_.get(), s);
+  fields_inverse(background_.get(), count_, m_, 1.0f, false, s);
+  check(hipStreamSynchronize(s), "set_background");
+  has_background_ = true;
+}
+
+// Fields of the sample holograms, their ratio to the background fields and the Rytov field.
+// The 1/N^2 scale and the (-1)^(x+y) output sign cancel in the ratio and are skipped.
+void Pipeline::front_end(const float* holo_dev, hipStream_t s, bool from_host) {
+  const std::size_t image = static_cast<std::size_t>(n_) * n_;
+  const std::size_t field = static_cast<std::size_t>(m_) * m_;
+  const int h = n_ / 2 + 1;
+  clock_.mark(kStart, s);
+  if (from_host) {
+    const hipStream_t c = copy_.get();
+    check(hipStreamWaitEvent(c, clock_.event(kStart).get(), 0), "hipStreamWaitEvent");
+    for (int k = 0; k < chunks_; ++k) {
+      const int first = k * chunk_;
+      const int valid = std::min(chunk_, count_ - first);
+      check(hipMemcpyAsync(holo_.get() + first * image, input_.get() + first * image,
+                           valid * image * sizeof(float), hipMemcpyHostToDevice, c),
+            "upload holograms");
+      chunk_done_[k].record(c);
+    }
+  }
+  for (int k = 0; k < chunks_; ++k) {
+    const int first = k * chunk_;
+    const int valid = std::min(chunk_, count_ - first);
+    if (from_host) check(hipStreamWaitEvent(s, chunk_done_[k].get(), 0), "hipStreamWaitEvent");
+    const float* src = (from_host ? holo_.get() : holo_dev) + first * image;
+    float2* half = half_.get() + static_cast<std::size_t>(first) * n_ * h;
+    hologram_spectra(src, valid, n_, half, s);
+    crop_fields(half, valid, n_, crop_, fields_.get() + first * field, s);
+  }
+  fields_inverse(fields_.get(), count_, m_, 1.0f, false, s);
+  clock_.mark(kFields, s);
+  ratio_phase(fields_.get(), background_.get(), static_cast<long>(count_) * field, psi_.get(),
+              logamp_.get(), s);
+  unwrapper_.rytov(psi_.get(), logamp_.get(), fields_.get(), s);
+  clock_.mark(kRytov, s);
+}
+
+void Pipeline::process() {
+  if (!has_background_) throw LogicError("Pipeline: set_background() first");
+  const auto t0 = std::chrono::steady_clock::now();
+  const hipStream_t s = compute_.get();
+  front_end(nullptr, s, true);
+  solver_.run(fields_.get(), s, &clock_, kSolver);
+  const std::size_t vol = static_cast<std::size_t>(solver_.nz()) * solver_.n() * solver_.n();
+  check(hipMemcpyAsync(output_.get(), solver_.index(), vol * sizeof(float), hipMemcpyDeviceToHost, s),
+        "download index");
+  check(hipMemcpyAsync(filled_.get(), solver_.filled_device(), sizeof(unsigned long long),
+                       hipMemcpyDeviceToHost, s),
+        "download filled");
+  clock_.mark(kDownloaded, s);
+  check(hipStreamSynchronize(s), "process");
+  collect(true);
+  timings_pending_ = false;
+  timings_.upload = elapsed_ms(clock_.event(kStart), chunk_done_.back());
+  timings_.total = since_ms(t0);
+}
+
+void Pipeline::process_device(const float* holo_dev, float* ri_dev, hipStream_t s) {
+  if (!has_background_) throw LogicError("Pipeline: set_background() first");
+  if (s == nullptr) s = compute_.get();
+  front_end(holo_dev, s, false);
+  solver_.run(fields_.get(), s, &clock_, kSolver);
+  const std::size_t vol = static_cast<std::size_t>(solver_.nz()) * solver_.n() * solver_.n();
+  check(hipMemcpyAsync(ri_dev, solver_.index(), vol * sizeof(float), hipMemcpyDeviceToDevice, s),
+        "copy index");
+  check(hipMemcpyAsync(filled_.get(), solver_.filled_device(), sizeof(unsigned long long),
+                       hipMemcpyDeviceToHost, s),
+        "download filled");
+  clock_.mark(kDownloaded, s);
+  timings_pending_ = true;
+}
+
+const qh_pipeline_timings& Pipeline::timings() {
+  if (timings_pending_) {
+    check(hipEventSynchronize(clock_.event(kDownloaded).get()), "timings");
+    collect(false);
+    timings_.upload = 0;
+    timings_.total = clock_.ms(kStart, kDownloaded);
+    timings_pending_ = false;
+  }
+  return timings_;
+}
+
+void Pipeline::collect(bool downloaded) {
+  const int o = kSolver;
+  timings_.fields = clock_.ms(kStart, kFields);
+  timings_.rytov = clock_.ms(kFields, kRytov);
+  timings_.fft2 = clock_.ms(kRytov, o + kFft2);
+  timings_.clear = clock_.ms(o + kFft2, o + kClear);
+  timings_.mapping = clock_.ms(o + kClear, o + kMapping);
+  timings_.normalize = clock_.ms(o + kMapping, o + kNormalize);
+  timings_.solve = clock_.ms(o + kNormalize, o + kSolve);
+  timings_.finish = clock_.ms(o + kSolve, o + kFinish);
+  timings_.download = downloaded ? clock_.ms(o + kFinish, kDownloaded) : 0.0;
+  timings_.fgp_ms = solver_.collect_fgp_ms();
+  timings_.fgp_steps = solver_.fgp_steps();
+}
+
+double Pipeline::filled() const {
+  const double vol = static_cast<double>(solver_.nz()) * solver_.n() * solver_.n();
+  return static_cast<double>(*filled_.get()) / vol;
+}
+
+}  // namespace qh

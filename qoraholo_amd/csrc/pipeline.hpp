@@ -1,0 +1,68 @@
+// Device-resident reconstruction pipeline: holograms in, refractive index out, with every
+// buffer, twiddle table and background field allocated once (include/qoraholo/pipeline.hpp of
+// the reference, src/cuda/pipeline.cu:12-161).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <vector>
+
+#include "../../include/qoraholo_hip.h"
+#include "runtime.hpp"
+#include "stages.hpp"
+
+namespace qh {
+
+class Pipeline {
+ public:
+  explicit Pipeline(const qh_pipeline_options& o);
+  void set_background(const float* host_backgrounds);
+  float* input() { return input_.get(); }
+  const float* output() const { return output_.get(); }
+  void process();
+  void process_device(const float* holo_dev, float* ri_dev, hipStream_t s);
+  double filled() const;
+  // Stage timings of the last call; after process_device they are read once its work is done.
+  const qh_pipeline_timings& timings();
+  double setup_ms() const { return setup_ms_; }
+  std::size_t device_bytes() const { return device_bytes_; }
+  int nz() const { return solver_.nz(); }
+  int n() const { return solver_.n(); }
+  void time_fgp(bool on) { solver_.time_fgp(on); }
+  bool has_background() const { return has_background_; }
+  void set_setup(double ms, std::size_t bytes) {
+    setup_ms_ = ms;
+    device_bytes_ = bytes;
+  }
+
+ private:
+  enum { kStart = 0, kFields, kRytov, kSolver, kDownloaded = kSolver + kOdtMarks, kMarks };
+  void front_end(const float* holo_dev, hipStream_t s, bool from_host);
+  void collect(bool downloaded);
+
+  int count_, n_, m_, chunk_, chunks_;
+  CropPlan crop_;
+  Stream compute_;
+  Stream copy_;
+  Unwrapper unwrapper_;
+  OdtSolver solver_;
+  DeviceBuffer<float> holo_;
+  DeviceBuffer<float2> half_;
+  DeviceBuffer<float2> background_;
+  DeviceBuffer<float2> fields_;
+  DeviceBuffer<float> psi_;
+  DeviceBuffer<float> logamp_;
+  PinnedBuffer<float> input_;
+  PinnedBuffer<float> output_;
+  PinnedBuffer<unsigned long long> filled_;
+  std::vector<Event> chunk_done_;
+  StageClock clock_;
+  qh_pipeline_timings timings_{};
+  double setup_ms_ = 0;
+  std::size_t device_bytes_ = 0;
+  bool has_background_ = false;
+  bool timings_pending_ = false;
+};
+
+}  // namespace qh

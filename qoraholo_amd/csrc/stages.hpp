@@ -1,0 +1,143 @@
+// Device stages of the reconstruction chain: field extraction, unwrapping and Rytov, and the
+// ODT solver. Each stage owns its device buffers; callers pass device pointers and a stream.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <memory>
+#include <vector>
+
+#include "common.hpp"
+#include "runtime.hpp"
+#include "tv.hpp"
+
+namespace qh {
+
+// Integer crop centre, radius and output side of field extraction (reference backends.hpp:20-25).
+struct CropPlan {
+  int cy;
+  int cx;
+  float radius;
+  int out;
+};
+
+// ---- field extraction (src/cpu/field.cpp:78-121 on the device) ----
+
+// Half spectra (count, n, n/2+1) of real holograms (count, n, n): row R2C then y transform.
+void hologram_spectra(const float* holo, int count, int n, float2* half, hipStream_t s);
+// Disc of radius r around the carrier into (count, m, m) with the carrier at m/2, times the
+// per-image sign (-1)^(cy+cx); bins with kx < 0 are read as conj(H(-ky, -kx)).
+void crop_fields(const float2* half, int count, int n, const CropPlan& p, float2* fields,
+                 hipStream_t s);
+// Inverse 2D transform of (count, m, m) fields times scale, optionally demodulated by
+// (-1)^(x+y) (the centre-origin output of the per-call path).
+void fields_inverse(float2* fields, int count, int m, float scale, bool demodulate, hipStream_t s);
+// psi = arg(s / b), logamp = log|s / b| with r = s conj(b) / |b|^2.
+void ratio_phase(const float2* sample, const float2* background, long total, float* psi,
+                 float* logamp, hipStream_t s);
+// psi = arg r, logamp = log|r| of a given ratio.
+void split_ratio(const float2* ratio, long total, float* psi, float* logamp, hipStream_t s);
+// Plain 2D transform of (count, n, n) complex images in place.
+void fft2_images(float2* data, int count, int n, bool inverse, hipStream_t s);
+
+// ---- unwrapping and Rytov (src/cpu/unwrap.cpp:11-82 on the device) ----
+
+class Unwrapper {
+ public:
+  Unwrapper(int count, int ny, int nx);
+  // Least-squares unwrap of psi into phi, congruent to psi modulo 2 pi.
+  void unwrap(const float* psi, float* phi, hipStream_t s);
+  // out = logamp + i (unwrap(psi) - 2 pi round(mean over the border / 2 pi)).
+  void rytov(const float* psi, const float* logamp, float2* out, hipStream_t s);
+  std::size_t bytes() const;
+
+ private:
+  void solve(const float* psi, hipStream_t s, bool border);
+  int count_, ny_, nx_;
+  int nblk_;
+  DeviceBuffer<float> phi_;
+  DeviceBuffer<double2> partial_;
+  DeviceBuffer<float2> shift_;  // per image: congruence constant c, border shift
+  DeviceBuffer<float> eig_y_;   // 2 cos(pi k / ny), host-evaluated in float
+  DeviceBuffer<float> eig_x_;
+};
+
+// ---- ODT solver (src/cpu/odt.cpp:124-242 on the device) ----
+
+struct OdtConfig {
+  int iterations = 0;
+  int solver = 0;        // 0: Gerchberg-Papoulis, 1: FISTA-TV
+  int gp_transform = 0;  // 0: r2c, 1: c2c
+  float tv_weight = 1e-2f;
+  int tv_inner = 10;
+  float wavelength = 0.532f;
+  float n_medium = 1.337f;
+};
+
+// Event marks the solver records when a clock is attached.
+enum OdtMark { kFft2 = 0, kClear, kMapping, kNormalize, kSolve, kFinish, kOdtMarks };
+
+class StageClock {
+ public:
+  explicit StageClock(int marks) : events_(marks) {}
+  void mark(int i, hipStream_t s) { events_[i].record(s); }
+  double ms(int a, int b) const { return elapsed_ms(events_[a], events_[b]); }
+  Event& event(int i) { return events_[i]; }
+
+ private:
+  std::vector<Event> events_;
+};
+
+class OdtSolver {
+ public:
+  OdtSolver(const OdtGrid& g, int count, const OdtConfig& c);
+  void set_illuminations(const float* illum_host);  // (count, 2) fy, fx
+  // fields: (count, n, n) complex on the device, overwritten by their spectra. The index volume
+  // (nz, n, n) is left in index(). Marks go to clock[first + OdtMark] when given.
+  void run(float2* fields, hipStream_t s, StageClock* clock = nullptr, int first = 0);
+  // Normalised k-space potential and hit counts after the mapping (for intermediate checks).
+  void map_only(float2* fields, hipStream_t s);
+  const float* index() const { return ri_.get(); }
+  const float2* potential() const { return potential_.get(); }
+  const std::uint16_t* counts() const { return counts_.get(); }
+  const unsigned long long* filled_device() const { return filled_.get(); }
+  std::size_t bytes() const;
+  int nz() const { return g_.nz; }
+  int n() const { return g_.n; }
+  // Sum of the FGP step durations of the last run and the number of steps, when timed.
+  double fgp_ms() const { return fgp_ms_; }
+  // Reads the FGP events of the last run; call after the stream has finished.
+  double collect_fgp_ms();
+  int fgp_steps() const { return fgp_steps_; }
+  void time_fgp(bool on) { time_fgp_ = on; }
+
+ private:
+  void map(float2* fields, hipStream_t s, StageClock* clock, int first, bool fold);
+  OdtGrid g_;
+  int count_;
+  OdtConfig c_;
+  bool real_;
+  bool tv_;
+  long vol_;
+  long half_;
+  float nm2_, inv_k02_, lambda_;
+  DeviceBuffer<float4> illum_;
+  DeviceBuffer<float2> potential_;
+  DeviceBuffer<std::uint16_t> counts_;
+  DeviceBuffer<float2> measured_;
+  DeviceBuffer<float2> a_;
+  DeviceBuffer<std::uint8_t> hits_;
+  DeviceBuffer<float2> e_;
+  DeviceBuffer<float> x_[3];
+  DeviceBuffer<float> ri_;
+  DeviceBuffer<unsigned long long> filled_;
+  std::unique_ptr<TvState> tv_state_;
+  std::vector<Event> fgp_events_;
+  bool time_fgp_ = false;
+  double fgp_ms_ = 0;
+  int fgp_steps_ = 0;
+};
+
+}  // namespace qh

@@ -1,0 +1,46 @@
+// FGP total-variation proximal step on the device.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+#include "runtime.hpp"
+
+namespace qh {
+
+// d[0, n) = v, by a kernel (safe inside captured graphs, unlike a memset node).
+void fill(float* d, long n, float v, hipStream_t s);
+
+// One fused FGP dual step over a dense (nz, ny, nx) volume: z = max(b - lambda L(r_in), 0)
+// at each voxel and its upper neighbours, g = L^T z, then the projected dual update of p and
+// the extrapolation r_out. p_in may equal p_out; r_in never equals r_out.
+void fgp_step(const float* b, const float* r_in, const float* p_in, float* p_out, float* r_out,
+              int nz, int ny, int nx, float lambda, float step, float momentum, hipStream_t s);
+
+// out = max(b - lambda L(p), 0); with to_index the refractive index of that potential.
+void tv_primal(const float* b, const float* p, int nz, int ny, int nx, float lambda, float* out,
+               bool to_index, float nm2, float inv_k02, hipStream_t s);
+
+// Dual state of the TV prox, kept across calls (FISTA warm start): two p buffers and two r
+// buffers of three components each.
+class TvState {
+ public:
+  TvState(int nz, int ny, int nx);
+  void reset(hipStream_t s);
+  // out = prox_{lambda TV, x >= 0}(b) by `inner` FGP steps from the current dual.
+  // Events t0 and t1, when given, bracket the FGP steps (not the final primal kernel).
+  void prox(const float* b, float* out, float lambda, int inner, hipStream_t s,
+            bool to_index = false, float nm2 = 0.0f, float inv_k02 = 0.0f, Event* t0 = nullptr,
+            Event* t1 = nullptr);
+  std::size_t bytes() const;
+
+ private:
+  int nz_, ny_, nx_;
+  long vol_;
+  DeviceBuffer<float> p_[2];
+  DeviceBuffer<float> r_[2];
+  int cur_ = 0;
+};
+
+}  // namespace qh

@@ -1,0 +1,53 @@
+import os
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+if str(ROOT) not in sys.path:
+    sys.path.insert(0, str(ROOT))
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device) and the built library")
+    config.addinivalue_line("markers", "slow: takes more than a few seconds on the CPU")
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    import oracle as o
+
+    o.lib()
+    return o
+
+
+@pytest.fixture(scope="session")
+def qh():
+    import qoraholo_amd as q
+
+    return q
+
+
+@pytest.fixture(scope="session")
+def device(qh):
+    """The HIP path must be there on a GPU run: a missing library or device fails the test."""
+    from qoraholo_amd import _native
+
+    _native.lib()
+    assert qh.hip_available(), "no HIP device visible"
+    return True
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.complex128)
+    b = np.asarray(b, np.complex128)
+    return float(np.sqrt(np.sum(np.abs(a - b) ** 2) / np.sum(np.abs(b) ** 2)))
+
+
+def contrast_error(a, b, nm):
+    """Relative L2 on the index contrast ri - n_m (the reference's tests/test_cuda.cpp:92-97)."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.sqrt(np.sum((a - b) ** 2) / np.sum((b - nm) ** 2)))

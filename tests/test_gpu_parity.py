@@ -1,0 +1,254 @@
+"""HIP path against the CPU oracle on the same seeded inputs, through the C-ABI.
+
+Tolerances: relative L2 of at most 5e-5 on every floating-point output (index contrast
+ri - n_m for volumes, as tests/test_cuda.cpp:92-97 of the reference measures it); Ewald hit
+counts and the filled fraction bit-exact. The cases follow the reference's tests/test_cuda.cpp
+and tests/test_pipeline.cpp.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+from conftest import contrast_error, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL = 5e-5
+GEOM = (0.532, 1.337, 1.2, 0.2)
+CELL = [(0.0, 0.4, -0.3, 2.5, 1.36), (0.0, 0.4, -0.3, 1.0, 1.38)]
+
+
+def geom(qh, g=GEOM):
+    return qh.Geometry(*g)
+
+
+def cell(qh):
+    return [qh.Sphere(*s) for s in CELL]
+
+
+# ---- transforms ----
+
+@pytest.mark.parametrize("shape", [(8, 16, 32), (6, 10, 12), (4, 96, 80), (3, 128, 256), (2, 512, 8)])
+def test_fft_c2c_matches_numpy(device, shape):
+    from qoraholo_amd import _native
+
+    rng = np.random.default_rng(0)
+    a = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+    for forward in (True, False):
+        got = a.copy()
+        _native.check(_native.lib().qh_hip_fft_c2c(_native.ptr(got), *shape, 7, int(forward), 1.0))
+        want = (np.fft.fftn(a.astype(np.complex128)) if forward
+                else np.fft.ifftn(a.astype(np.complex128)) * a.size)
+        assert rel_l2(got, want) < 2e-6, (shape, forward)
+
+
+@pytest.mark.parametrize("shape", [(16, 32, 64), (12, 20, 30), (64, 64, 128), (128, 128, 128)])
+def test_real_3d_transforms_match_numpy(device, shape):
+    from qoraholo_amd import _native
+
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(shape).astype(np.float32)
+    h = np.empty(shape[:2] + (shape[2] // 2 + 1,), np.complex64)
+    _native.check(_native.lib().qh_hip_rfft3(_native.ptr(x), *shape, _native.ptr(h)))
+    assert rel_l2(h, np.fft.rfftn(x.astype(np.float64))) < 2e-6
+    back = np.empty_like(x)
+    _native.check(_native.lib().qh_hip_irfft3(_native.ptr(h), *shape, 1.0 / x.size, _native.ptr(back)))
+    assert rel_l2(back, x) < 2e-6
+
+
+# ---- field extraction (tests/test_cuda.cpp:37-66) ----
+
+def test_field_extraction_matches_oracle(device, oracle, qh):
+    illum = qh.circular_scan(5, 0.8, geom(qh))
+    hg = geom(qh, (0.532, 1.337, 1.2, 0.05))
+    ratio = qh.forward_fields(cell(qh), illum, hg, 128)
+    holo = qh.synthesize_holograms(ratio, illum, hg, carrier=(5.0, 5.0), noise=0.02, seed=1)
+    got = qh.extract_fields(holo, center=(32.0, 32.0), radius=16.0, output_size=64)
+    want = oracle.extract_fields(holo, 32, 32, 16.0, 64)
+    assert rel_l2(got, want) < TOL
+
+
+def test_field_extraction_across_zero_kx(device, oracle, qh):
+    illum = qh.circular_scan(3, 0.5, geom(qh))
+    hg = geom(qh, (0.532, 1.337, 1.2, 0.05))
+    df = 1.0 / (128 * 0.05)
+    ratio = qh.forward_fields(cell(qh), illum, hg, 128)
+    holo = qh.synthesize_holograms(ratio, illum, hg, carrier=(-30.0 * df, 4.0 * df), noise=0.02,
+                                   seed=1)
+    got = qh.extract_fields(holo, center=(-30.0, 4.0), radius=12.0, output_size=32)
+    want = oracle.extract_fields(holo, -30, 4, 12.0, 32)
+    assert rel_l2(got, want) < TOL
+
+
+# ---- unwrap and Rytov (tests/test_cuda.cpp:68-72, tests/test_unwrap.cpp) ----
+
+def smooth_phase(nz, ny, nx):
+    from test_oracle import smooth_phase as sp
+
+    return sp(nz, ny, nx)
+
+
+def test_unwrap_matches_oracle_and_known_answer(device, oracle, qh):
+    from test_oracle import wrap
+
+    truth = smooth_phase(2, 96, 128)
+    psi = wrap(truth)
+    got = qh.unwrap(psi)
+    want = oracle.unwrap(psi)
+    assert rel_l2(got, want) < TOL
+    for z in range(2):
+        offset = float(got[z, 0, 0]) - float(truth[z, 0, 0])
+        assert abs(math.remainder(offset, 2 * math.pi)) < 1e-3
+        assert np.max(np.abs(got[z] - truth[z] - offset)) < 2e-3
+
+
+def test_rytov_matches_oracle(device, oracle, qh):
+    illum = qh.circular_scan(5, 0.8, geom(qh))
+    ratio = qh.forward_fields(cell(qh), illum, geom(qh), 64)
+    got = qh.rytov(ratio)
+    want = oracle.rytov(ratio)
+    assert rel_l2(got, want) < TOL
+    assert rel_l2(got.imag, want.imag) < TOL
+
+
+def test_rytov_border_known_answer(device, qh):
+    phase = smooth_phase(1, 64, 64) + np.float32(4 * math.pi)
+    ratio = (np.float32(0.8) * np.exp(1j * phase.astype(np.float64))).astype(np.complex64)
+    r = qh.rytov(ratio)
+    assert np.max(np.abs(r.real - np.log(np.float32(0.8)))) < 1e-5
+    assert np.max(np.abs(r.imag - (phase - np.float32(4 * math.pi)))) < 2e-3
+
+
+# ---- Ewald mapping and reconstruction (tests/test_cuda.cpp:74-137) ----
+
+@pytest.fixture(scope="module")
+def rytov_fields(oracle, qh):
+    illum = qh.circular_scan(25, 0.9, geom(qh))
+    fields = oracle.rytov(qh.forward_fields(cell(qh), illum, geom(qh), 64))
+    return illum, fields
+
+
+def test_kspace_counts_bit_exact(device, oracle, qh, rytov_fields):
+    illum, fields = rytov_fields
+    pot, counts, filled = qh.kspace(fields, illum, geom(qh))
+    _, want_filled, want_pot, want_w = oracle.reconstruct(fields, illum, GEOM, intermediates=True)
+    assert np.array_equal(counts.astype(np.float32), want_w)
+    assert filled == want_filled
+    assert rel_l2(pot, want_pot) < TOL
+
+
+@pytest.mark.parametrize("case", [
+    dict(iterations=5, gp_transform="r2c"),
+    dict(iterations=5, gp_transform="r2c", mapping="scatter"),
+    dict(iterations=5, gp_transform="c2c"),
+    dict(iterations=0),
+    dict(iterations=10, solver="tv"),
+    dict(iterations=3, solver="tv", nz=48),
+])
+def test_reconstruction_matches_oracle(device, oracle, qh, rytov_fields, case):
+    illum, fields = rytov_fields
+    got, filled = qh.reconstruct(fields, illum, geom(qh), return_filled=True, **case)
+    ocase = {k: v for k, v in case.items() if k != "mapping"}
+    want, want_filled = oracle.reconstruct(fields, illum, GEOM, **ocase)
+    assert contrast_error(got, want, GEOM[1]) < TOL
+    assert filled == want_filled
+
+
+def test_tv_prox_partial_tiles(device, oracle):
+    # 62 x 60 x 60 leaves partial tiles along every axis of the FGP kernel.
+    from qoraholo_amd import _native
+
+    rng = np.random.default_rng(5)
+    b = (rng.standard_normal((62, 60, 60)) * 0.1 + 0.05).astype(np.float32)
+    lam = 0.02
+    want = oracle.tv_prox(b, lam, 10)
+    got = np.empty_like(b)
+    _native.check(_native.lib().qh_hip_tv_prox(_native.ptr(b), 62, 60, 60, lam, 10,
+                                               _native.ptr(got)))
+    assert np.max(np.abs(got - want)) < 1e-6
+
+
+def test_tv_timings_count_fgp_steps(device, qh, rytov_fields):
+    illum, fields = rytov_fields
+    t = {}
+    qh.reconstruct(fields, illum, geom(qh), iterations=4, solver="tv", tv_inner=6, timings=t)
+    assert t["fgp_steps"] == 24
+    assert t["fgp_ms"] > 0
+    assert t["solve"] >= t["fgp_ms"]
+
+
+# ---- Pipeline (tests/test_pipeline.cpp) ----
+
+class PipelineData:
+    def __init__(self, qh, holo=128, angles=9, iterations=5, solver="gp"):
+        self.holo_geom = qh.Geometry(0.532, 1.337, 1.2, 0.1)
+        self.geom = qh.Geometry(0.532, 1.337, 1.2, 0.2)
+        self.illum = qh.circular_scan(angles, 0.9, self.geom)
+        ratio = qh.forward_fields(cell(qh), self.illum, self.holo_geom, holo)
+        df = 1.0 / (holo * 0.1)
+        carrier = (36.0 * df, 36.0 * df)
+        self.holo = qh.synthesize_holograms(ratio, self.illum, self.holo_geom, carrier=carrier,
+                                            noise=0.01, seed=3)
+        self.bg = qh.synthesize_holograms(np.ones_like(ratio), self.illum, self.holo_geom,
+                                          carrier=carrier, noise=0.01, seed=3)
+        self.kw = dict(center=(36.0, 36.0), radius=30.0, output_size=holo // 2,
+                       iterations=iterations, solver=solver)
+        self.count = angles
+        self.size = holo
+
+    def reference(self, oracle):
+        return oracle.pipeline(self.holo, self.bg, 36, 36, 30.0, self.size // 2, self.illum, GEOM,
+                               iterations=self.kw["iterations"], solver=self.kw["solver"])
+
+
+@pytest.mark.parametrize("chunks", [1, 4])
+def test_pipeline_matches_oracle_and_reuses_buffers(device, oracle, qh, chunks):
+    d = PipelineData(qh)
+    want, want_filled = d.reference(oracle)
+    p = qh.Pipeline(d.count, d.size, illuminations=d.illum, geometry=d.geom,
+                    upload_chunks=chunks, **d.kw)
+    p.set_background(d.bg)
+    first = p.run(d.holo)
+    assert contrast_error(first, want, GEOM[1]) < TOL
+    assert p.filled == want_filled
+    second = p.run(d.holo)
+    assert np.array_equal(first, second)
+    assert p.device_bytes > 0
+    t = p.timings
+    assert t["total"] > 0 and t["solve"] > 0
+
+
+def test_pipeline_needs_background(device, qh):
+    d = PipelineData(qh, holo=64, angles=3)
+    p = qh.Pipeline(d.count, d.size, illuminations=d.illum, geometry=d.geom, **d.kw)
+    with pytest.raises(qh.LogicError):
+        p.run(d.holo)
+
+
+def test_pipeline_tv_matches_oracle(device, oracle, qh):
+    d = PipelineData(qh, iterations=5, solver="tv")
+    want, want_filled = d.reference(oracle)
+    p = qh.Pipeline(d.count, d.size, illuminations=d.illum, geometry=d.geom, **d.kw)
+    p.set_background(d.bg)
+    got = p.run(d.holo)
+    assert contrast_error(got, want, GEOM[1]) < TOL
+    assert p.filled == want_filled
+
+
+def test_bead_workload_gp20_matches_oracle(device, oracle, qh):
+    # 49 x 256^2 holograms -> 128^3 with 20 GP iterations, the CPU-runnable configuration.
+    from qoraholo_amd import workloads
+
+    w = workloads.bead()
+    g = w.geometry
+    p = qh.Pipeline(49, 256, w.center, w.illuminations, g, radius=w.radius, output_size=128,
+                    iterations=20)
+    p.set_background(w.backgrounds)
+    got = p.run(w.holograms)
+    want, want_filled = oracle.pipeline(w.holograms, w.backgrounds, 71, 71, w.radius, 128,
+                                        w.illuminations, (g.wavelength, g.n_medium, g.na, g.pixel),
+                                        iterations=20)
+    assert contrast_error(got, want, g.n_medium) < TOL
+    assert p.filled == want_filled

@@ -1,0 +1,126 @@
+"""Host-side checks that run without a GPU: the C-ABI library loads and exports what
+include/qoraholo_hip.h declares, argument validation follows the reference's rules and raises
+the exceptions its bindings raise, and the host helpers (crop plan, grid, synthesiser)."""
+
+import ctypes
+import math
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def declared_symbols():
+    text = (ROOT / "include" / "qoraholo_hip.h").read_text()
+    return sorted(set(re.findall(r"\b(qh_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    from qoraholo_amd import _native
+
+    lib = _native.lib()
+    names = declared_symbols()
+    assert len(names) >= 25
+    for name in names:
+        assert hasattr(lib, name), name
+    assert sorted(_native.symbols()) == names
+
+
+def test_unknown_backend_is_rejected(qh):
+    with pytest.raises(ValueError, match="backend"):
+        qh.unwrap(np.zeros((8, 8), np.float32), backend="cuda")
+    with pytest.raises(ValueError, match="backend"):
+        qh.reconstruct(np.zeros((1, 8, 8), np.complex64), [[0, 0]], qh.Geometry(), backend="cpu")
+
+
+def test_validation_errors_are_value_errors(qh):
+    g = qh.Geometry(0.532, 1.337, 1.2, 0.2)
+    with pytest.raises(ValueError, match="square with an even side"):
+        qh.extract_fields(np.zeros((1, 10, 12), np.float32), center=(2, 2))
+    with pytest.raises(ValueError, match="crop radius"):
+        qh.extract_fields(np.zeros((1, 64, 64), np.float32), center=(10, 10), radius=40,
+                          output_size=64)
+    with pytest.raises(ValueError, match="output_size"):
+        qh.extract_fields(np.zeros((1, 64, 64), np.float32), center=(10, 10), output_size=30 + 1)
+    with pytest.raises(ValueError, match="even"):
+        qh.unwrap(np.zeros((1, 9, 8), np.float32))
+    with pytest.raises(ValueError, match="illumination frequency"):
+        qh.reconstruct(np.zeros((1, 16, 16), np.complex64), [[3.0, 0.0]], g)
+    with pytest.raises(ValueError, match="nz must be even"):
+        qh.reconstruct(np.zeros((1, 16, 16), np.complex64), [[0.0, 0.0]], g, nz=15)
+    with pytest.raises(ValueError, match="2\\^a 3\\^b 5\\^c"):
+        qh.reconstruct(np.zeros((1, 14, 14), np.complex64), [[0.0, 0.0]], g)
+    with pytest.raises(ValueError, match="one illumination per field"):
+        qh.reconstruct(np.zeros((2, 16, 16), np.complex64), [[0.0, 0.0]], g)
+    with pytest.raises(ValueError, match="unknown solver"):
+        qh.reconstruct(np.zeros((1, 16, 16), np.complex64), [[0.0, 0.0]], g, solver="admm")
+
+
+def test_pipeline_validation(qh):
+    g = qh.Geometry(0.532, 1.337, 1.2, 0.2)
+    il = qh.circular_scan(4, 0.9, g)
+    with pytest.raises(ValueError, match="carrier"):
+        qh.Pipeline(4, 64, None, il, g)
+    with pytest.raises(ValueError, match="upload_chunks"):
+        qh.Pipeline(4, 64, (16, 16), il, g, output_size=32, upload_chunks=5)
+    with pytest.raises(ValueError, match="hologram_size"):
+        qh.Pipeline(4, 63, (16, 16), il, g, output_size=32)
+
+
+def test_crop_plan_rounds_and_defaults(qh):
+    from qoraholo_amd import _crop
+
+    p = _crop(256, (70.5, 70.5), 32.0, 128)
+    assert (p.cy, p.cx, p.out) == (71, 71, 128)
+    p = _crop(256, (-30.4, 4.6), 0.0, 0)
+    assert (p.cy, p.cx, p.out) == (-30, 5, 256)
+    assert p.radius == np.float32(math.hypot(np.float32(-30.4), np.float32(4.6))) / np.float32(3)
+
+
+def test_grid_constants(qh):
+    from qoraholo_amd import _grid
+
+    g = _grid(256, 256, qh.Geometry(0.532, 1.337, 1.2, 0.11))
+    f = np.float32
+    fm = f(1.337) / f(0.532)
+    assert g.fm2 == fm * fm
+    assert g.df == f(1) / (f(256) * f(0.11))
+    assert g.gain == f(4) * f(math.pi) / f(0.11)
+
+
+def test_noise_stream_is_seeded_gaussian(qh):
+    from qoraholo_amd.phantom import gaussian_noise
+
+    a = gaussian_noise(0, 0.5, 200000)
+    b = gaussian_noise(0, 0.5, 200000)
+    c = gaussian_noise(1, 0.5, 1000)
+    assert np.array_equal(a, b) and not np.array_equal(a[:1000], c)
+    assert abs(float(a.mean())) < 0.005
+    assert abs(float(a.std()) - 0.5) < 0.005
+
+
+def test_hologram_synthesis_closed_form(qh):
+    # Without a sample (ratio 1) the hologram of a normal-incidence wave and a unit reference is
+    # |1 + exp(2 pi i c.r)|^2 = 2 + 2 cos(2 pi c.r).
+    g = qh.Geometry(0.532, 1.337, 1.2, 0.1)
+    h = qh.synthesize_holograms(np.ones((1, 32, 32), np.complex64), [[0, 0]], g, carrier=(1.0, 2.0))
+    r = (np.arange(32) - 16) * np.float32(0.1)
+    want = 2 + 2 * np.cos(2 * np.pi * (1.0 * r[:, None] + 2.0 * r[None, :]))
+    assert np.max(np.abs(h[0] - want)) < 1e-5
+
+
+def test_cell_shapes_are_nested(qh):
+    from qoraholo_amd.workloads import cell_shapes
+
+    body, nucleus, v1, v2 = cell_shapes(0)
+    for s in (nucleus, v1, v2):
+        # The farthest point of each sphere from the body centre stays inside the body.
+        for d in np.eye(3):
+            for sign in (-1, 1):
+                p = np.array([s.z, s.y, s.x]) + sign * s.radius * d
+                assert (p[0] / body.az) ** 2 + (p[1] / body.ay) ** 2 + (p[2] / body.ax) ** 2 < 1
+    for a, b in ((nucleus, v1), (nucleus, v2), (v1, v2)):
+        assert math.dist((a.z, a.y, a.x), (b.z, b.y, b.x)) > a.radius + b.radius
