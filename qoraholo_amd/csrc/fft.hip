@@ -279,7 +279,8 @@ __global__ void __launch_bounds__(kThreads) k_z_roundtrip_restore(ZArgs a) {
 }
 
 int contig_lines_per_block(int n) { return std::max(1, std::min(64, 2048 / n)); }
-int strided_lines_per_block(int n) { return n <= 256 ? 16 : std::max(1, std::min(16, 4096 / n)); }
+// Lines per block keep the two LDS buffers within 32 KiB (dynamic LDS stays below 64 KiB).
+int strided_lines_per_block(int n) { return std::max(1, std::min(16, 2048 / n)); }
 
 }  // namespace
 
@@ -303,7 +304,8 @@ FftFactors fft_factors(int n) {
 }
 
 bool fft_supported(int n) {
-  if (n < 1) return false;
+  // One line of up to 2048 complex values fits the LDS budget of the transform kernels.
+  if (n < 1 || n > 2048) return false;
   for (int r : {2, 3, 5}) {
     while (n % r == 0) n /= r;
   }

@@ -278,10 +278,9 @@ void Unwrapper::solve(const float* psi, hipStream_t s, bool border) {
   for (int pass = 0; pass < 2; ++pass) {
     const bool forward = pass == 0;
     if (!forward) {
-      ColArgs c{phi, static_cast<long>(count_) * nx_, ny_, nx_, 16, fft_factors(ny_),
-                fft_twiddles(ny_), dct_twiddles(ny_), eig_y_.get(), eig_x_.get(),
-                1.0f / static_cast<float>(ny_)};
-      if (ny_ > 256) c.lines_per_block = std::max(1, 4096 / ny_);
+      ColArgs c{phi, static_cast<long>(count_) * nx_, ny_, nx_,
+                std::max(1, std::min(16, 2048 / ny_)), fft_factors(ny_), fft_twiddles(ny_),
+                dct_twiddles(ny_), eig_y_.get(), eig_x_.get(), 1.0f / static_cast<float>(ny_)};
       const std::size_t lds = (2 * ny_ + 2 * static_cast<std::size_t>(c.lines_per_block) * ny_) * sizeof(float2);
       const unsigned grid = static_cast<unsigned>((c.lines + c.lines_per_block - 1) / c.lines_per_block);
       hipLaunchKernelGGL(k_dct_cols_poisson, dim3(grid), dim3(kThreads), lds, s, c);
