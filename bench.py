@@ -1,0 +1,195 @@
+"""Benchmark: 256^3 FISTA-TV reconstructions per second on one MI355X.
+
+Workload (the reference README's cell at full scale): 49 off-axis holograms of 512^2 fp32 with
+their background holograms -> fields cropped to 256^2 -> ratio, DCT unwrap, Rytov -> Ewald
+mapping into 256^3 -> 20 FISTA-TV iterations with 10 FGP dual steps each (TV weight 1e-2) ->
+index volume. Synthetic inputs from the package's synthesiser; backgrounds are set once per
+geometry (Pipeline.set_background), and each step reconstructs from holograms already in HBM
+into a volume in HBM.
+
+Prints one JSON line: throughput (recon/s), per-step latency percentiles, the roofline of the
+dominant kernel (the FGP dual step, timed with HIP events on the stream it runs on), a CPU
+baseline (the oracle, a C++/OpenMP restatement of the reference's CPU path, timed for one
+reconstruction of the same inputs) and the relative error of the HIP volume against it.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "256³ FISTA-TV recon/s + p50/p99 latency (served vs plain), 1 MI355X"
+HBM_PEAK_GBS = 8000.0
+N = 256
+ANGLES = 49
+ITERATIONS = 20
+INNER = 10
+TV_WEIGHT = 1e-2
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def fgp_traffic():
+    """HBM bytes per FGP launch measured by rocprofv3 (FETCH_SIZE + WRITE_SIZE, scaled by the
+    copy-kernel calibration), from the committed summary when it exists."""
+    path = ROOT / "profiles" / "fgp_step_256_hbm_traffic.txt"
+    if not path.exists():
+        return None
+    m = re.search(r"traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", path.read_text())
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(w):
+    """Times the oracle (the C++/OpenMP restatement of the reference's CPU path) on the same
+    inputs for one whole reconstruction, and returns its volume for the parity check."""
+    import oracle
+
+    g = w.geometry
+    geom = (g.wavelength, g.n_medium, g.na, g.pixel)
+    cy, cx = int(round(w.center[0])), int(round(w.center[1]))
+    t0 = time.perf_counter()
+    ri, _ = oracle.pipeline(w.holograms, w.backgrounds, cy, cx, w.radius, N, w.illuminations, geom,
+                            iterations=ITERATIONS, solver="tv", tv_weight=TV_WEIGHT,
+                            tv_inner=INNER)
+    secs = time.perf_counter() - t0
+    log(f"cpu baseline: one reconstruction in {secs:.2f} s")
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    return {
+        "value": round(1.0 / secs, 4),
+        "unit": "recon/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": (f"one full reconstruction of the same workload by the oracle (C++/OpenMP "
+                   f"restatement of the reference CPU path, own FFT), {secs:.2f} s"),
+    }, ri
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true",
+                    help="skip the oracle run (CPU baseline and full-size parity)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: the reconstruction path runs on one device and the "
+              f"multi-GPU design (requests sharded across devices) is not built; got --gpus "
+              f"{args.gpus}", file=sys.stderr)
+        return 2
+
+    import torch
+
+    import qoraholo_amd as qh
+    from qoraholo_amd import workloads
+
+    if not qh.hip_available():
+        raise SystemExit("no HIP device or libqoraholo_hip.so missing")
+    t0 = time.perf_counter()
+    w = workloads.readme_cell(n=N, angles=ANGLES, seed=0)
+    log(f"workload generated in {time.perf_counter() - t0:.1f} s")
+
+    p = qh.Pipeline(ANGLES, 2 * N, w.center, w.illuminations, w.geometry, radius=w.radius,
+                    output_size=N, iterations=ITERATIONS, solver="tv", tv_weight=TV_WEIGHT,
+                    tv_inner=INNER)
+    p.set_background(w.backgrounds)
+    dev = torch.device("cuda:0")
+    holo = torch.from_numpy(w.holograms).to(dev)
+    out = torch.empty((N, N, N), dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+
+    for _ in range(args.warmup):
+        p.run_device(holo, out)
+    torch.cuda.synchronize()
+    log(f"warmup done ({args.warmup} steps)")
+
+    # Throughput: K reconstructions back to back on the pipeline's stream.
+    p.time_fgp(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        p.run_device(holo, out)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    t = p.timings
+    ms_per_step = 1000.0 * elapsed / args.steps
+    log(f"timed {args.steps} steps: {ms_per_step:.2f} ms per reconstruction")
+
+    # Latency of the plain path: each request submitted and waited for on its own.
+    lat = []
+    for _ in range(args.steps):
+        s0 = time.perf_counter()
+        p.run_device(holo, out)
+        torch.cuda.synchronize()
+        lat.append(1000.0 * (time.perf_counter() - s0))
+    p50, p99 = np.percentile(lat, [50, 99])
+
+    vol = N * N * N
+    fgp_bytes = 13 * vol * 4  # read b, r(3), p(3); write p(3), r(3): fp32 volumes
+    fgp_avg_ms = t["fgp_ms"] / max(1, t["fgp_steps"])
+    achieved = fgp_bytes / (fgp_avg_ms * 1e-3) / 1e9
+    roofline = {
+        "bound": "hbm",
+        "kernel": "k_fgp_step",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": fgp_traffic(),
+        "launch_ms": round(fgp_avg_ms, 5),
+        "algorithmic_bytes": fgp_bytes,
+    }
+    stages = {k: round(v, 3) for k, v in t.items() if k not in ("fgp_steps",)}
+
+    result = {
+        "metric": METRIC,
+        "value": round(1000.0 / ms_per_step, 3),
+        "unit": "recon/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": ("readme_cell: 49 x 512^2 fp32 holograms + backgrounds -> 256^2 fields -> "
+                         "256^3 FISTA-TV 20x10, tv_weight 1e-2, inputs resident in HBM"),
+            "holograms": [ANGLES, 2 * N, 2 * N],
+            "volume": [N, N, N],
+            "solver": f"fista-tv {ITERATIONS}x{INNER}",
+        },
+        "latency_ms": {"plain": {"p50": round(float(p50), 3), "p99": round(float(p99), 3)},
+                       "served": None},
+        "stage_ms": stages,
+        "roofline": roofline,
+    }
+
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"], want = cpu_baseline(w)
+        got = out.cpu().numpy().astype(np.float64)
+        nm = w.geometry.n_medium
+        err = float(np.sqrt(np.sum((got - want) ** 2) / np.sum((want.astype(np.float64) - nm) ** 2)))
+        raw = float(np.sqrt(np.sum((got - want) ** 2) / np.sum(want.astype(np.float64) ** 2)))
+        result["parity"] = {"contrast_rel_l2_vs_oracle": err, "ri_rel_l2_vs_oracle": raw,
+                            "tolerance": 5e-5}
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

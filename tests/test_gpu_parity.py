@@ -221,7 +221,7 @@ def test_pipeline_matches_oracle_and_reuses_buffers(device, oracle, qh, chunks):
 
 
 def test_pipeline_needs_background(device, qh):
-    d = PipelineData(qh, holo=64, angles=3)
+    d = PipelineData(qh, angles=3)
     p = qh.Pipeline(d.count, d.size, illuminations=d.illum, geometry=d.geom, **d.kw)
     with pytest.raises(qh.LogicError):
         p.run(d.holo)
