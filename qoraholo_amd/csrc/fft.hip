@@ -9,6 +9,7 @@
 
 #include "common.hpp"
 #include "fft.hpp"
+#include "fft4s.hpp"
 #include "runtime.hpp"
 
 namespace qh {
@@ -278,6 +279,241 @@ __global__ void __launch_bounds__(kThreads) k_z_roundtrip_restore(ZArgs a) {
   }
 }
 
+// ---- four-step kernels for lengths 32 .. 512 ----
+
+template <int N1, int N2>
+struct Shape4 {
+  static constexpr int N = N1 * N2;
+  static constexpr int T = N1 > N2 ? N1 : N2;
+  static constexpr int LINES = 256 / T;
+};
+
+template <int N1, int N2, bool LINE_FAST>
+__global__ void __launch_bounds__(256) k_fft4s_lines(LinesArgs a) {
+  using S = Shape4<N1, N2>;
+  __shared__ float2 tw[S::N];
+  __shared__ float2 t[t_size<N1, N2, S::LINES, LINE_FAST>()];
+  const int tid = static_cast<int>(threadIdx.x);
+  for (int i = tid; i < S::N; i += 256) tw[i] = a.tw[i];
+  const int line = LINE_FAST ? tid % S::LINES : tid / S::T;
+  const int j = LINE_FAST ? tid / S::LINES : tid % S::T;
+  const long l = static_cast<long>(blockIdx.x) * S::LINES + line;
+  const bool valid = l < a.g.count;
+  const long base = valid ? line_base(l, a.g) : 0;
+  float2 v[N2];
+  float2 w[N1];
+#pragma unroll
+  for (int n2 = 0; n2 < N2; ++n2) {
+    v[n2] = (valid && j < N1) ? a.in[base + static_cast<long>(j + N1 * n2) * a.g.stride]
+                              : float2{0.0f, 0.0f};
+  }
+  __syncthreads();
+  fft_4step<N1, N2, S::LINES, LINE_FAST>(v, w, j, line, t, tw, a.inverse != 0);
+  if (valid && j < N2) {
+    const int lm = a.alt_mod > 0 ? static_cast<int>(l % a.alt_mod) : 0;
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) {
+      const int k = N2 * k1 + j;
+      float2 val = w[k1];
+      if (a.scale != 1.0f) val = cscale(val, a.scale);
+      if (a.alt_mod > 0 && ((k + lm) & 1)) val = {-val.x, -val.y};
+      a.out[base + static_cast<long>(k) * a.g.stride] = val;
+    }
+  }
+}
+
+// z lines of the half spectrum: forward transform, restore, inverse transform.
+template <int N1, int N2>
+__global__ void __launch_bounds__(256) k_fft4s_zrt(ZArgs a) {
+  using S = Shape4<N1, N2>;
+  __shared__ float2 tw[S::N];
+  __shared__ float2 t[t_size<N1, N2, S::LINES, true>()];
+  const int tid = static_cast<int>(threadIdx.x);
+  for (int i = tid; i < S::N; i += 256) tw[i] = a.tw[i];
+  const int line = tid % S::LINES;
+  const int j = tid / S::LINES;
+  const long l = static_cast<long>(blockIdx.x) * S::LINES + line;
+  const bool valid = l < a.lines;
+  float2 v[N2];
+  float2 w[N1];
+#pragma unroll
+  for (int n2 = 0; n2 < N2; ++n2) {
+    v[n2] = (valid && j < N1) ? a.e[static_cast<long>(j + N1 * n2) * a.plane + l] : float2{0.0f, 0.0f};
+  }
+  __syncthreads();
+  fft_4step<N1, N2, S::LINES, true>(v, w, j, line, t, tw, false);
+  if (valid && j < N2) {
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) {
+      const long g = static_cast<long>(N2 * k1 + j) * a.plane + l;
+      const float2 d = a.a[g];
+      hermitian_restore(d.x, d.y, a.hits[g], w[k1].x, w[k1].y);
+    }
+  }
+  // Thread j now holds X[j + N2 k1]: the input layout of the transform with the factors swapped.
+  float2 u[N2];
+  fft_4step<N2, N1, S::LINES, true>(w, u, j, line, t, tw, true);
+  if (valid && j < N1) {
+#pragma unroll
+    for (int k1 = 0; k1 < N2; ++k1) a.e[static_cast<long>(N1 * k1 + j) * a.plane + l] = u[k1];
+  }
+}
+
+// Real rows of n = 2 N1 N2 through the packed complex transform, then the split into bins
+// 0 .. n/2 (see k_rfft_rows).
+template <int N1, int N2>
+__global__ void __launch_bounds__(256) k_rfft4s_rows(RowsArgs a) {
+  using S = Shape4<N1, N2>;
+  constexpr int M = S::N;
+  __shared__ float2 twm[M];
+  __shared__ float2 twn[M + 1];
+  __shared__ float2 t[t_size<N1, N2, S::LINES, false>()];
+  const int tid = static_cast<int>(threadIdx.x);
+  for (int i = tid; i < M; i += 256) twm[i] = a.tw[2 * i];
+  for (int i = tid; i <= M; i += 256) twn[i] = a.tw[i];
+  const int line = tid / S::T;
+  const int j = tid % S::T;
+  const long row = static_cast<long>(blockIdx.x) * S::LINES + line;
+  const bool valid = row < a.rows;
+  float2 v[N2];
+  float2 w[N1];
+#pragma unroll
+  for (int n2 = 0; n2 < N2; ++n2) {
+    float2 z{0.0f, 0.0f};
+    if (valid && j < N1) {
+      const long at = row * a.x_pitch + 2 * (j + N1 * n2);
+      z = *reinterpret_cast<const float2*>(a.x + at);
+      if (a.op.in == RealIn::nonneg) {
+        z.x = z.x < 0.0f ? 0.0f : z.x;
+        z.y = z.y < 0.0f ? 0.0f : z.y;
+      } else if (a.op.in == RealIn::momentum) {
+        const float2 q = *reinterpret_cast<const float2*>(a.x2 + at);
+        z.x = z.x + a.op.a * (z.x - q.x);
+        z.y = z.y + a.op.a * (z.y - q.y);
+      }
+    }
+    v[n2] = z;
+  }
+  __syncthreads();
+  fft_4step<N1, N2, S::LINES, false>(v, w, j, line, t, twm, false);
+  float2* zrow = t + line * (M + 1);
+  if (j < N2) {
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) zrow[N2 * k1 + j] = w[k1];
+  }
+  __syncthreads();
+  if (valid) {
+    for (int k = j; k <= M; k += S::T) {
+      const float2 zk = zrow[k == M ? 0 : k];
+      const float2 zm = cconj(zrow[k == 0 ? 0 : M - k]);
+      const float2 ev = {0.5f * (zk.x + zm.x), 0.5f * (zk.y + zm.y)};
+      const float2 d = {0.5f * (zk.x - zm.x), 0.5f * (zk.y - zm.y)};
+      const float2 od = cmul(twn[k], d);
+      const float2 o = {od.y, -od.x};
+      a.c[row * a.c_pitch + k] = (k == 0 || k == M) ? float2{ev.x + o.x, 0.0f} : cadd(ev, o);
+    }
+  }
+}
+
+template <int N1, int N2>
+__global__ void __launch_bounds__(256) k_irfft4s_rows(RowsArgs a) {
+  using S = Shape4<N1, N2>;
+  constexpr int M = S::N;
+  __shared__ float2 twm[M];
+  __shared__ float2 twn[M + 1];
+  __shared__ float2 t[t_size<N1, N2, S::LINES, false>()];
+  const int tid = static_cast<int>(threadIdx.x);
+  for (int i = tid; i < M; i += 256) twm[i] = a.tw[2 * i];
+  for (int i = tid; i <= M; i += 256) twn[i] = a.tw[i];
+  const int line = tid / S::T;
+  const int j = tid % S::T;
+  const long row = static_cast<long>(blockIdx.x) * S::LINES + line;
+  const bool valid = row < a.rows;
+  float2* xrow = t + line * (M + 1);
+  for (int k = j; k <= M; k += S::T) xrow[k] = valid ? a.c[row * a.c_pitch + k] : float2{0.0f, 0.0f};
+  __syncthreads();
+  float2 v[N2];
+  float2 w[N1];
+#pragma unroll
+  for (int n2 = 0; n2 < N2; ++n2) {
+    const int k = j + N1 * n2;
+    float2 z{0.0f, 0.0f};
+    if (j < N1) {
+      float2 xk = xrow[k];
+      float2 xm = xrow[M - k];
+      if (k == 0) {
+        xk.y = 0.0f;
+        xm.y = 0.0f;
+      }
+      const float2 s = cadd(xk, cconj(xm));
+      const float2 d = csub(xk, cconj(xm));
+      const float2 wd = cmul(cconj(twn[k]), d);
+      z = cadd(s, float2{-wd.y, wd.x});
+    }
+    v[n2] = z;
+  }
+  __syncthreads();
+  fft_4step<N1, N2, S::LINES, false>(v, w, j, line, t, twm, true);
+  if (valid && j < N2) {
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) {
+      const int m = N2 * k1 + j;
+      float u0 = w[k1].x * a.op.a;
+      float u1 = w[k1].y * a.op.a;
+      if (a.op.out == RealOut::index) {
+        u0 = index_from_potential(u0, 0.0f, a.op.nm2, a.op.inv_k02);
+        u1 = index_from_potential(u1, 0.0f, a.op.nm2, a.op.inv_k02);
+      }
+      *reinterpret_cast<float2*>(a.xo + row * a.x_pitch + 2 * m) = float2{u0, u1};
+    }
+  }
+}
+
+// Runs F<N1, N2> for the lengths with a four-step kernel; false for the others.
+template <template <int, int> class F, class... Args>
+bool with_four_step(int n, Args&&... args) {
+  switch (n) {
+    case 32: F<8, 4>::run(args...); return true;
+    case 64: F<8, 8>::run(args...); return true;
+    case 128: F<16, 8>::run(args...); return true;
+    case 256: F<16, 16>::run(args...); return true;
+    case 512: F<32, 16>::run(args...); return true;
+    default: return false;
+  }
+}
+
+template <int N1, int N2>
+struct LaunchLines {
+  static void run(const LinesArgs& a, bool contig, hipStream_t s) {
+    const unsigned grid = static_cast<unsigned>((a.g.count + Shape4<N1, N2>::LINES - 1) / Shape4<N1, N2>::LINES);
+    if (contig) {
+      hipLaunchKernelGGL((k_fft4s_lines<N1, N2, false>), dim3(grid), dim3(256), 0, s, a);
+    } else {
+      hipLaunchKernelGGL((k_fft4s_lines<N1, N2, true>), dim3(grid), dim3(256), 0, s, a);
+    }
+  }
+};
+
+template <int N1, int N2>
+struct LaunchZrt {
+  static void run(const ZArgs& z, hipStream_t s) {
+    const unsigned grid = static_cast<unsigned>((z.lines + Shape4<N1, N2>::LINES - 1) / Shape4<N1, N2>::LINES);
+    hipLaunchKernelGGL((k_fft4s_zrt<N1, N2>), dim3(grid), dim3(256), 0, s, z);
+  }
+};
+
+template <int N1, int N2>
+struct LaunchRows {
+  static void run(const RowsArgs& a, bool inverse, hipStream_t s) {
+    const unsigned grid = static_cast<unsigned>((a.rows + Shape4<N1, N2>::LINES - 1) / Shape4<N1, N2>::LINES);
+    if (inverse) {
+      hipLaunchKernelGGL((k_irfft4s_rows<N1, N2>), dim3(grid), dim3(256), 0, s, a);
+    } else {
+      hipLaunchKernelGGL((k_rfft4s_rows<N1, N2>), dim3(grid), dim3(256), 0, s, a);
+    }
+  }
+};
+
 int contig_lines_per_block(int n) { return std::max(1, std::min(64, 2048 / n)); }
 // Lines per block keep the two LDS buffers within 32 KiB (dynamic LDS stays below 64 KiB).
 int strided_lines_per_block(int n) { return std::max(1, std::min(16, 2048 / n)); }
@@ -327,6 +563,11 @@ void fft_lines(const float2* in, float2* out, int n, const Lines& l, bool invers
                int alt_mod, hipStream_t s) {
   if (l.count == 0) return;
   LinesArgs a{in, out, l, n, 0, fft_factors(n), fft_twiddles(n), inverse ? 1 : 0, scale, alt_mod};
+  if (l.stride != 1) require(l.inner_stride == 1, "fft: strided lines must be adjacent");
+  if (with_four_step<LaunchLines>(n, a, l.stride == 1, s)) {
+    check_launch("fft_lines");
+    return;
+  }
   if (l.stride == 1) {
     a.lines_per_block = contig_lines_per_block(n);
     const std::size_t lds = (n + 2 * static_cast<std::size_t>(a.lines_per_block) * n) * sizeof(float2);
@@ -349,6 +590,10 @@ void rfft_rows(const float* x, const float* x2, long x_pitch, float2* out, long 
   const int m = n / 2;
   RowsArgs a{x, x2, nullptr, x_pitch, out, out_pitch, rows, n, contig_lines_per_block(m),
              fft_factors(m), fft_twiddles(n), op};
+  if (x_pitch % 2 == 0 && with_four_step<LaunchRows>(m, a, false, s)) {
+    check_launch("rfft_rows");
+    return;
+  }
   const std::size_t lds =
       (n + m + static_cast<std::size_t>(a.rows_per_block) * (2 * m + 1)) * sizeof(float2);
   const unsigned grid = static_cast<unsigned>((rows + a.rows_per_block - 1) / a.rows_per_block);
@@ -363,6 +608,10 @@ void irfft_rows(const float2* in, long in_pitch, float* x, long x_pitch, long ro
   const int m = n / 2;
   RowsArgs a{nullptr, nullptr, x, x_pitch, const_cast<float2*>(in), in_pitch, rows, n,
              contig_lines_per_block(m), fft_factors(m), fft_twiddles(n), op};
+  if (x_pitch % 2 == 0 && with_four_step<LaunchRows>(m, a, true, s)) {
+    check_launch("irfft_rows");
+    return;
+  }
   const std::size_t lds =
       (n + m + static_cast<std::size_t>(a.rows_per_block) * (2 * m + 1)) * sizeof(float2);
   const unsigned grid = static_cast<unsigned>((rows + a.rows_per_block - 1) / a.rows_per_block);
@@ -375,6 +624,10 @@ void z_roundtrip_restore(float2* e, const float2* a, const std::uint8_t* hits, i
   const long plane = static_cast<long>(ny) * h;
   ZArgs z{e, a, hits, plane, plane, nz, strided_lines_per_block(nz), fft_factors(nz),
           fft_twiddles(nz)};
+  if (with_four_step<LaunchZrt>(nz, z, s)) {
+    check_launch("z_roundtrip_restore");
+    return;
+  }
   const std::size_t lds = (nz + 2 * static_cast<std::size_t>(z.lines_per_block) * nz) * sizeof(float2);
   const unsigned grid = static_cast<unsigned>((plane + z.lines_per_block - 1) / z.lines_per_block);
   hipLaunchKernelGGL(k_z_roundtrip_restore, dim3(grid), dim3(kThreads), lds, s, z);

@@ -1,7 +1,12 @@
 // Total-variation proximal step by fast gradient projection on the dual (FGP), the inner loop
-// of FISTA-TV. One kernel per dual step: a 64 x 8 tile of (x, y) columns marches along z, the
-// dual r of two planes (with a one-voxel halo) and the primal point z of two planes live in
-// LDS, so every voxel's b, p and r are read from HBM once per step and p, r written once.
+// of FISTA-TV, on gfx950.
+//
+// One wavefront per block owns a strip of TY rows of the volume, each lane VX consecutive x
+// positions of every row (a whole 256-wide row is 64 lanes x 4 floats, loaded as float4), and
+// marches along z through a chunk of planes. y neighbours are in the lane's own registers, x
+// neighbours one lane away (shuffles), z neighbours in the registers of the previous plane, so
+// every voxel's b, p and r are read once per step and p and r written once; only one halo row
+// above and below the strip (and one plane at the chunk start) is read twice.
 #include "common.hpp"
 #include "runtime.hpp"
 #include "tv.hpp"
@@ -10,10 +15,7 @@ namespace qh {
 
 namespace {
 
-constexpr int kTx = 64;
-constexpr int kTy = 8;
-constexpr int kZChunk = 32;
-constexpr int kThreadsTv = kTx * kTy;
+constexpr int kZChunk = 16;
 
 struct TvArgs {
   const float* b;     // primal data, dense (nz, ny, nx)
@@ -28,123 +30,236 @@ struct TvArgs {
   float momentum;
 };
 
-__device__ __forceinline__ long at3(const TvArgs& a, int z, int y, int x) {
-  return (static_cast<long>(z) * a.ny + y) * a.nx + x;
+// Block b runs tile t: with a grid that is a multiple of 8, blocks b, b+8, b+16, ... (which the
+// dispatcher places on the same XCD) get consecutive tiles, so the halo rows and the planes
+// shared by neighbouring tiles are read through the same L2.
+__device__ __forceinline__ int xcd_tile() {
+  const int nb = static_cast<int>(gridDim.x);
+  const int b = static_cast<int>(blockIdx.x);
+  return (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
 }
 
-__global__ void __launch_bounds__(kThreadsTv) k_fgp_step(TvArgs a) {
-  constexpr int hx = kTx + 2;
-  constexpr int hy = kTy + 2;
-  // r on two planes, local y, x in [-1, kT]; z on two planes, local y, x in [0, kT].
-  __shared__ float sr[2][3][hy][hx];
-  __shared__ float sz[2][kTy + 1][kTx + 1];
-  const int x0 = static_cast<int>(blockIdx.x) * kTx;
-  const int y0 = static_cast<int>(blockIdx.y) * kTy;
-  const int z0 = static_cast<int>(blockIdx.z) * kZChunk;
-  const int z1 = min(z0 + kZChunk, a.nz);
-  const int tid = static_cast<int>(threadIdx.x);
-  const int tx = tid % kTx;
-  const int ty = tid / kTx;
-
-  const auto load = [&](int slot, int z) {
-    for (int i = tid; i < hx * hy; i += kThreadsTv) {
-      const int lx = i % hx;
-      const int ly = i / hx;
-      const int x = x0 - 1 + lx;
-      const int y = y0 - 1 + ly;
-      const bool in = x >= 0 && x < a.nx && y >= 0 && y < a.ny && z >= 0 && z < a.nz;
-      const long g = in ? at3(a, z, y, x) : 0;
+template <int VX>
+__device__ __forceinline__ void load_row(const float* base, const TvArgs& a, int z, int y, int xb,
+                                         bool vec, float (&out)[VX]) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) sr[slot][c][ly][lx] = in ? a.r_in[c * a.vol + g] : 0.0f;
-    }
-  };
-  // z on plane z from r on that plane (slot) and on the plane below (below); the additions run
-  // in the order of the negative divergence L(r) = sum_a r_a(i) - r_a(i - e_a), axes z, y, x.
-  const auto primal = [&](int out, int slot, int below, int z) {
-    for (int i = tid; i < (kTx + 1) * (kTy + 1); i += kThreadsTv) {
-      const int lx = i % (kTx + 1);
-      const int ly = i / (kTx + 1);
-      const int x = x0 + lx;
-      const int y = y0 + ly;
-      float zv = 0.0f;
-      if (x < a.nx && y < a.ny && z < a.nz) {
-        const int cx = lx + 1;
-        const int cy = ly + 1;
-        float s = 0.0f;
-        if (z < a.nz - 1) s += sr[slot][0][cy][cx];
-        if (z > 0) s -= sr[below][0][cy][cx];
-        if (y < a.ny - 1) s += sr[slot][1][cy][cx];
-        if (y > 0) s -= sr[slot][1][cy - 1][cx];
-        if (x < a.nx - 1) s += sr[slot][2][cy][cx];
-        if (x > 0) s -= sr[slot][2][cy][cx - 1];
-        zv = tv_primal(a.b[at3(a, z, y, x)], a.lambda, s);
+  for (int v = 0; v < VX; ++v) out[v] = 0.0f;
+  if (z < 0 || z >= a.nz || y < 0 || y >= a.ny) return;
+  const float* p = base + (static_cast<long>(z) * a.ny + y) * a.nx + xb;
+  if constexpr (VX % 4 == 0) {
+    if (vec && xb + VX <= a.nx) {
+#pragma unroll
+      for (int v = 0; v < VX; v += 4) {
+        const float4 q = *reinterpret_cast<const float4*>(p + v);
+        out[v] = q.x;
+        out[v + 1] = q.y;
+        out[v + 2] = q.z;
+        out[v + 3] = q.w;
       }
-      sz[out][ly][lx] = zv;
+      return;
     }
-  };
-
-  int cur = 0;
-  int zc = 0;
-  load(1, z0 - 1);
-  load(cur, z0);
-  __syncthreads();
-  primal(zc, cur, 1, z0);
-  const int x = x0 + tx;
-  const int y = y0 + ty;
-  const bool inside = x < a.nx && y < a.ny;
-  for (int z = z0; z < z1; ++z) {
-    const int next = 1 - cur;
-    const long g = inside ? at3(a, z, y, x) : 0;
-    float pv[3];
+  } else if constexpr (VX % 2 == 0) {
+    if (vec && xb + VX <= a.nx) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) pv[c] = inside ? a.p_in[c * a.vol + g] : 0.0f;
-    __syncthreads();  // plane z - 1 in `next` and z-slot 1 - zc are no longer read
-    load(next, z + 1);
-    __syncthreads();
-    primal(1 - zc, next, cur, z + 1);
-    __syncthreads();
-    if (inside) {
-      const float c = sz[zc][ty][tx];
-      float gr[3];
-      gr[0] = z < a.nz - 1 ? c - sz[1 - zc][ty][tx] : 0.0f;
-      gr[1] = y < a.ny - 1 ? c - sz[zc][ty + 1][tx] : 0.0f;
-      gr[2] = x < a.nx - 1 ? c - sz[zc][ty][tx + 1] : 0.0f;
-      float rv[3];
-#pragma unroll
-      for (int cc = 0; cc < 3; ++cc) rv[cc] = sr[cur][cc][ty + 1][tx + 1];
-      tv_dual_step(gr, a.step, a.momentum, pv, rv);
-#pragma unroll
-      for (int cc = 0; cc < 3; ++cc) {
-        a.p_out[cc * a.vol + g] = pv[cc];
-        a.r_out[cc * a.vol + g] = rv[cc];
+      for (int v = 0; v < VX; v += 2) {
+        const float2 q = *reinterpret_cast<const float2*>(p + v);
+        out[v] = q.x;
+        out[v + 1] = q.y;
       }
+      return;
     }
-    cur = next;
-    zc = 1 - zc;
+  }
+#pragma unroll
+  for (int v = 0; v < VX; ++v) {
+    if (xb + v < a.nx) out[v] = p[v];
   }
 }
 
-// out = max(b - lambda L(p), 0), optionally followed by the refractive index of out.
-__global__ void k_tv_primal(const float* b, const float* p, long vol, int nz, int ny, int nx,
-                            float lambda, float* out, int to_index, float nm2, float inv_k02) {
-  const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= vol) return;
-  const int x = static_cast<int>(i % nx);
-  const int y = static_cast<int>((i / nx) % ny);
-  const int z = static_cast<int>(i / (static_cast<long>(nx) * ny));
-  const long plane = static_cast<long>(nx) * ny;
-  const float* pz = p;
-  const float* py = p + vol;
-  const float* px = p + 2 * vol;
-  float s = 0.0f;
-  if (z < nz - 1) s += pz[i];
-  if (z > 0) s -= pz[i - plane];
-  if (y < ny - 1) s += py[i];
-  if (y > 0) s -= py[i - nx];
-  if (x < nx - 1) s += px[i];
-  if (x > 0) s -= px[i - 1];
-  const float v = tv_primal(b[i], lambda, s);
-  out[i] = to_index ? index_from_potential(v, 0.0f, nm2, inv_k02) : v;
+template <int VX>
+__device__ __forceinline__ void store_row(float* base, const TvArgs& a, int z, int y, int xb,
+                                          bool vec, const float (&in)[VX]) {
+  if (z >= a.nz || y >= a.ny) return;
+  float* p = base + (static_cast<long>(z) * a.ny + y) * a.nx + xb;
+  if constexpr (VX % 4 == 0) {
+    if (vec && xb + VX <= a.nx) {
+#pragma unroll
+      for (int v = 0; v < VX; v += 4) {
+        *reinterpret_cast<float4*>(p + v) = float4{in[v], in[v + 1], in[v + 2], in[v + 3]};
+      }
+      return;
+    }
+  } else if constexpr (VX % 2 == 0) {
+    if (vec && xb + VX <= a.nx) {
+#pragma unroll
+      for (int v = 0; v < VX; v += 2) *reinterpret_cast<float2*>(p + v) = float2{in[v], in[v + 1]};
+      return;
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < VX; ++v) {
+    if (xb + v < a.nx) p[v] = in[v];
+  }
+}
+
+// Primal point z = max(b - lambda L(r), 0) of plane zp for the strip's rows y0..y0+TY (the
+// last one is the halo row below the strip). R holds r at plane zp, ry_top the y component of
+// r on row y0 - 1, rz_below the z component on plane zp - 1. The additions run in the order of
+// the negative divergence L(r)(i) = sum_a r_a(i) - r_a(i - e_a), axes z, y, x.
+template <int VX, int TY>
+__device__ __forceinline__ void primal_plane(const TvArgs& a, int zp, int y0, int xb,
+                                             const float (&R)[3][TY + 1][VX],
+                                             const float (&ry_top)[VX],
+                                             const float (&rz_below)[TY + 1][VX],
+                                             const float (&B)[TY + 1][VX],
+                                             float (&Z)[TY + 1][VX]) {
+#pragma unroll
+  for (int j = 0; j <= TY; ++j) {
+    const int y = y0 + j;
+    const float left = __shfl_up(R[2][j][VX - 1], 1);
+#pragma unroll
+    for (int v = 0; v < VX; ++v) {
+      const int x = xb + v;
+      float s = 0.0f;
+      if (zp < a.nz - 1) s += R[0][j][v];
+      if (zp > 0) s -= rz_below[j][v];
+      if (y < a.ny - 1) s += R[1][j][v];
+      if (y > 0) s -= (j == 0 ? ry_top[v] : R[1][j - 1][v]);
+      if (x < a.nx - 1) s += R[2][j][v];
+      if (x > 0) s -= (v == 0 ? left : R[2][j][v - 1]);
+      Z[j][v] = (y < a.ny && x < a.nx && zp < a.nz) ? tv_primal(B[j][v], a.lambda, s) : 0.0f;
+    }
+  }
+}
+
+template <int VX, int TY>
+__global__ void __launch_bounds__(64) k_fgp_step(TvArgs a, int chunks) {
+  const int t = xcd_tile();
+  const int y0 = (t / chunks) * TY;
+  const int z0 = (t % chunks) * kZChunk;
+  const int z1 = min(z0 + kZChunk, a.nz);
+  const int xb = static_cast<int>(threadIdx.x) * VX;
+  const bool vec = (a.nx % 4) == 0;
+
+  float r[3][TY + 1][VX];   // r at plane z, rows y0..y0+TY
+  float zc[TY + 1][VX];     // primal at plane z
+  float rn[3][TY + 1][VX];  // r at plane z + 1
+  float zn[TY + 1][VX];
+  float bb[TY + 1][VX];
+  float ry_top[VX];
+  float rz_below[TY + 1][VX];
+
+  // Warm-up: the primal on plane z0 needs r_z on plane z0 - 1.
+#pragma unroll
+  for (int j = 0; j <= TY; ++j) {
+    load_row<VX>(a.r_in, a, z0 - 1, y0 + j, xb, vec, rz_below[j]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) load_row<VX>(a.r_in + c * a.vol, a, z0, y0 + j, xb, vec, r[c][j]);
+    load_row<VX>(a.b, a, z0, y0 + j, xb, vec, bb[j]);
+  }
+  load_row<VX>(a.r_in + a.vol, a, z0, y0 - 1, xb, vec, ry_top);
+  primal_plane<VX, TY>(a, z0, y0, xb, r, ry_top, rz_below, bb, zc);
+
+  for (int z = z0; z < z1; ++z) {
+    float pv[3][TY][VX];
+#pragma unroll
+    for (int j = 0; j < TY; ++j) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) load_row<VX>(a.p_in + c * a.vol, a, z, y0 + j, xb, vec, pv[c][j]);
+    }
+#pragma unroll
+    for (int j = 0; j <= TY; ++j) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) load_row<VX>(a.r_in + c * a.vol, a, z + 1, y0 + j, xb, vec, rn[c][j]);
+      load_row<VX>(a.b, a, z + 1, y0 + j, xb, vec, bb[j]);
+    }
+    load_row<VX>(a.r_in + a.vol, a, z + 1, y0 - 1, xb, vec, ry_top);
+    primal_plane<VX, TY>(a, z + 1, y0, xb, rn, ry_top, r[0], bb, zn);
+
+#pragma unroll
+    for (int j = 0; j < TY; ++j) {
+      const int y = y0 + j;
+      const float right = __shfl_down(zc[j][0], 1);
+      float po[3][VX];
+      float ro[3][VX];
+#pragma unroll
+      for (int v = 0; v < VX; ++v) {
+        const int x = xb + v;
+        const float c = zc[j][v];
+        float g[3];
+        g[0] = z < a.nz - 1 ? c - zn[j][v] : 0.0f;
+        g[1] = y < a.ny - 1 ? c - zc[j + 1][v] : 0.0f;
+        g[2] = x < a.nx - 1 ? c - (v == VX - 1 ? right : zc[j][v + 1]) : 0.0f;
+        float p3[3] = {pv[0][j][v], pv[1][j][v], pv[2][j][v]};
+        float r3[3] = {r[0][j][v], r[1][j][v], r[2][j][v]};
+        tv_dual_step(g, a.step, a.momentum, p3, r3);
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) {
+          po[cc][v] = p3[cc];
+          ro[cc][v] = r3[cc];
+        }
+      }
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc) {
+        store_row<VX>(a.p_out + cc * a.vol, a, z, y, xb, vec, po[cc]);
+        store_row<VX>(a.r_out + cc * a.vol, a, z, y, xb, vec, ro[cc]);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j <= TY; ++j) {
+#pragma unroll
+      for (int v = 0; v < VX; ++v) {
+        zc[j][v] = zn[j][v];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) r[c][j][v] = rn[c][j][v];
+      }
+    }
+  }
+}
+
+// out = max(b - lambda L(p), 0) (optionally its refractive index), same strip layout.
+template <int VX, int TY>
+__global__ void __launch_bounds__(64) k_tv_primal(TvArgs a, int chunks, float* out, int to_index,
+                                                  float nm2, float inv_k02) {
+  const int t = xcd_tile();
+  const int y0 = (t / chunks) * TY;
+  const int z0 = (t % chunks) * kZChunk;
+  const int z1 = min(z0 + kZChunk, a.nz);
+  const int xb = static_cast<int>(threadIdx.x) * VX;
+  const bool vec = (a.nx % 4) == 0;
+  float p[3][TY + 1][VX];
+  float pz_below[TY + 1][VX];
+  float py_top[VX];
+  float bb[TY + 1][VX];
+  float x3[TY + 1][VX];
+#pragma unroll
+  for (int j = 0; j < TY; ++j) load_row<VX>(a.p_in, a, z0 - 1, y0 + j, xb, vec, pz_below[j]);
+#pragma unroll
+  for (int v = 0; v < VX; ++v) pz_below[TY][v] = 0.0f;
+  for (int z = z0; z < z1; ++z) {
+#pragma unroll
+    for (int j = 0; j < TY; ++j) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) load_row<VX>(a.p_in + c * a.vol, a, z, y0 + j, xb, vec, p[c][j]);
+      load_row<VX>(a.b, a, z, y0 + j, xb, vec, bb[j]);
+    }
+#pragma unroll
+    for (int v = 0; v < VX; ++v) {
+      p[0][TY][v] = p[1][TY][v] = p[2][TY][v] = 0.0f;
+      bb[TY][v] = 0.0f;
+    }
+    load_row<VX>(a.p_in + a.vol, a, z, y0 - 1, xb, vec, py_top);
+    primal_plane<VX, TY>(a, z, y0, xb, p, py_top, pz_below, bb, x3);
+#pragma unroll
+    for (int j = 0; j < TY; ++j) {
+      float o[VX];
+#pragma unroll
+      for (int v = 0; v < VX; ++v) {
+        o[v] = to_index ? index_from_potential(x3[j][v], 0.0f, nm2, inv_k02) : x3[j][v];
+        pz_below[j][v] = p[0][j][v];
+      }
+      store_row<VX>(out, a, z, y0 + j, xb, vec, o);
+    }
+  }
 }
 
 __global__ void k_fill(float* d, long n, float v) {
@@ -160,20 +275,57 @@ void fill(float* d, long n, float v, hipStream_t s) {
   check_launch("fill");
 }
 
+namespace {
+
+constexpr int kRows = 2;
+
+int lanes_x(int nx) {
+  const int vx = (nx + 63) / 64;
+  if (vx <= 1) return 1;
+  if (vx <= 2) return 2;
+  if (vx <= 4) return 4;
+  if (vx <= 6) return 6;
+  if (vx <= 8) return 8;
+  throw InvalidArgument("tv: volume side above 512");
+}
+
+dim3 strip_grid(int ny, int nz, int* chunks) {
+  *chunks = (nz + kZChunk - 1) / kZChunk;
+  return dim3(static_cast<unsigned>(((ny + kRows - 1) / kRows) * *chunks));
+}
+
+}  // namespace
+
 void fgp_step(const float* b, const float* r_in, const float* p_in, float* p_out, float* r_out,
               int nz, int ny, int nx, float lambda, float step, float momentum, hipStream_t s) {
   TvArgs a{b, r_in, p_in, p_out, r_out, static_cast<long>(nz) * ny * nx, nz, ny, nx,
            lambda, step, momentum};
-  const dim3 grid((nx + kTx - 1) / kTx, (ny + kTy - 1) / kTy, (nz + kZChunk - 1) / kZChunk);
-  hipLaunchKernelGGL(k_fgp_step, grid, dim3(kThreadsTv), 0, s, a);
+  int chunks = 0;
+  const dim3 grid = strip_grid(ny, nz, &chunks);
+  switch (lanes_x(nx)) {
+    case 1: hipLaunchKernelGGL((k_fgp_step<1, kRows>), grid, dim3(64), 0, s, a, chunks); break;
+    case 2: hipLaunchKernelGGL((k_fgp_step<2, kRows>), grid, dim3(64), 0, s, a, chunks); break;
+    case 4: hipLaunchKernelGGL((k_fgp_step<4, kRows>), grid, dim3(64), 0, s, a, chunks); break;
+    case 6: hipLaunchKernelGGL((k_fgp_step<6, kRows>), grid, dim3(64), 0, s, a, chunks); break;
+    default: hipLaunchKernelGGL((k_fgp_step<8, kRows>), grid, dim3(64), 0, s, a, chunks); break;
+  }
   check_launch("fgp_step");
 }
 
 void tv_primal(const float* b, const float* p, int nz, int ny, int nx, float lambda, float* out,
                bool to_index, float nm2, float inv_k02, hipStream_t s) {
-  const long vol = static_cast<long>(nz) * ny * nx;
-  hipLaunchKernelGGL(k_tv_primal, dim3(blocks_for(vol)), dim3(256), 0, s, b, p, vol, nz, ny, nx,
-                     lambda, out, to_index ? 1 : 0, nm2, inv_k02);
+  TvArgs a{b, nullptr, p, nullptr, nullptr, static_cast<long>(nz) * ny * nx, nz, ny, nx,
+           lambda, 0.0f, 0.0f};
+  int chunks = 0;
+  const dim3 grid = strip_grid(ny, nz, &chunks);
+  const int ti = to_index ? 1 : 0;
+  switch (lanes_x(nx)) {
+    case 1: hipLaunchKernelGGL((k_tv_primal<1, kRows>), grid, dim3(64), 0, s, a, chunks, out, ti, nm2, inv_k02); break;
+    case 2: hipLaunchKernelGGL((k_tv_primal<2, kRows>), grid, dim3(64), 0, s, a, chunks, out, ti, nm2, inv_k02); break;
+    case 4: hipLaunchKernelGGL((k_tv_primal<4, kRows>), grid, dim3(64), 0, s, a, chunks, out, ti, nm2, inv_k02); break;
+    case 6: hipLaunchKernelGGL((k_tv_primal<6, kRows>), grid, dim3(64), 0, s, a, chunks, out, ti, nm2, inv_k02); break;
+    default: hipLaunchKernelGGL((k_tv_primal<8, kRows>), grid, dim3(64), 0, s, a, chunks, out, ti, nm2, inv_k02); break;
+  }
   check_launch("tv_primal");
 }
 

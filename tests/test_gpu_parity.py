@@ -30,7 +30,8 @@ def cell(qh):
 
 # ---- transforms ----
 
-@pytest.mark.parametrize("shape", [(8, 16, 32), (6, 10, 12), (4, 96, 80), (3, 128, 256), (2, 512, 8)])
+@pytest.mark.parametrize("shape", [(8, 16, 32), (6, 10, 12), (4, 96, 80), (3, 128, 256), (2, 512, 8),
+                                   (256, 4, 8), (64, 256, 2), (2, 64, 512)])
 def test_fft_c2c_matches_numpy(device, shape):
     from qoraholo_amd import _native
 
@@ -153,6 +154,18 @@ def test_reconstruction_matches_oracle(device, oracle, qh, rytov_fields, case):
     ocase = {k: v for k, v in case.items() if k != "mapping"}
     want, want_filled = oracle.reconstruct(fields, illum, GEOM, **ocase)
     assert contrast_error(got, want, GEOM[1]) < TOL
+    assert filled == want_filled
+
+
+def test_tv_at_256_matches_oracle(device, oracle, qh):
+    # The full-size transforms (256-point z lines, 128-point packed rows) and FGP strips.
+    g = (0.532, 1.337, 1.2, 0.11)
+    illum = qh.circular_scan(9, 0.9, geom(qh, g))
+    fields = oracle.rytov(qh.forward_fields(cell(qh), illum, geom(qh, g), 256))
+    got, filled = qh.reconstruct(fields, illum, geom(qh, g), iterations=2, solver="tv",
+                                 return_filled=True)
+    want, want_filled = oracle.reconstruct(fields, illum, g, iterations=2, solver="tv")
+    assert contrast_error(got, want, g[1]) < TOL
     assert filled == want_filled
 
 
