@@ -76,6 +76,23 @@ def cpu_baseline(w):
     }, ri
 
 
+def fgp_traffic_run(qh, torch) -> int:
+    """5 FGP steps on a 256^3 volume (plus the prox's fill and primal kernels) and one copy of a
+    256 MiB buffer (reads and writes exactly 256 MiB) as the counter calibration."""
+    from qoraholo_amd import _native
+
+    rng = np.random.default_rng(0)
+    b = (rng.standard_normal((N, N, N)) * 0.01).astype(np.float32)
+    x = np.empty_like(b)
+    _native.check(_native.lib().qh_hip_tv_prox(_native.ptr(b), N, N, N, 0.5, 5, _native.ptr(x)))
+    src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    log("fgp traffic run: 5 FGP steps at 256^3 and a 268435456-byte copy")
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,6 +100,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true",
                     help="skip the oracle run (CPU baseline and full-size parity)")
+    ap.add_argument("--fgp-traffic", action="store_true",
+                    help="run only 5 FGP steps at 256^3 and a 256 MiB device copy, for the "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE runs that calibrate HBM traffic")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the reconstruction path runs on one device and the "
@@ -94,6 +114,9 @@ def main() -> int:
 
     import qoraholo_amd as qh
     from qoraholo_amd import workloads
+
+    if args.fgp_traffic:
+        return fgp_traffic_run(qh, torch)
 
     if not qh.hip_available():
         raise SystemExit("no HIP device or libqoraholo_hip.so missing")
