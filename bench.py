@@ -93,6 +93,99 @@ def fgp_traffic_run(qh, torch) -> int:
     return 0
 
 
+def pct(v, q):
+    return round(float(np.percentile(v, q)), 3)
+
+
+def serving_comparison(qh, w, plain_requests=3, serial_requests=5, served_requests=16, seed=0):
+    """Requests from host memory to host memory on the same workload:
+    - plain: the per-call chain a user of the reference's API runs (extract_fields for sample
+      and background, field_ratio on the host, rytov, reconstruct), serially;
+    - pipeline: Pipeline.run, serially (background fields resident);
+    - served: the Server, a burst of requests submitted at once (throughput), then a Poisson
+      stream at 80% of that throughput (latency under load)."""
+    g = w.geometry
+    kw = dict(iterations=ITERATIONS, solver="tv", tv_weight=TV_WEIGHT, tv_inner=INNER)
+    crop = dict(center=w.center, radius=w.radius, output_size=N)
+    out = {}
+
+    def plain_once():
+        ratio = qh.field_ratio(qh.extract_fields(w.holograms, **crop),
+                               qh.extract_fields(w.backgrounds, **crop))
+        return qh.reconstruct(qh.rytov(ratio), w.illuminations, g, **kw)
+
+    plain_once()
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(plain_requests):
+        s0 = time.perf_counter()
+        ri_plain = plain_once()
+        lat.append(1000.0 * (time.perf_counter() - s0))
+    out["plain"] = {"recon_s": round(plain_requests / (time.perf_counter() - t0), 3),
+                    "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "requests": plain_requests}
+    log(f"plain per-call: {out['plain']}")
+
+    p = qh.Pipeline(ANGLES, 2 * N, w.center, w.illuminations, g, radius=w.radius, output_size=N,
+                    **kw)
+    p.set_background(w.backgrounds)
+    p.run(w.holograms)
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(serial_requests):
+        s0 = time.perf_counter()
+        ri_pipe = p.run(w.holograms)
+        lat.append(1000.0 * (time.perf_counter() - s0))
+    out["pipeline"] = {"recon_s": round(serial_requests / (time.perf_counter() - t0), 3),
+                       "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
+                       "requests": serial_requests}
+    log(f"serial pipeline: {out['pipeline']}")
+    del p
+
+    server = qh.Server(max_slots=4)
+    gid = server.add_geometry(ANGLES, 2 * N, w.center, w.illuminations, g, w.backgrounds,
+                              radius=w.radius, output_size=N, **kw)
+    for rid in [server.submit(gid, w.holograms) for _ in range(4)]:
+        server.wait(rid)
+    server.reset_stats()
+    t0 = time.perf_counter()
+    ids = [server.submit(gid, w.holograms) for _ in range(served_requests)]
+    results = [server.wait(rid) for rid in ids]
+    burst_s = served_requests / (time.perf_counter() - t0)
+    st = server.stats()
+    ri_served = results[-1][0]
+    out["served_burst"] = {"recon_s": round(burst_s, 3), "p50_ms": round(st["p50_ms"], 3),
+                           "p99_ms": round(st["p99_ms"], 3), "requests": served_requests,
+                           "slots": st["slots"], "device_bytes": st["device_bytes"]}
+    log(f"served burst: {out['served_burst']}")
+
+    rate = 0.8 * burst_s
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, served_requests)
+    server.reset_stats()
+    ids = []
+    t0 = time.perf_counter()
+    for gap in gaps:
+        t0 += gap
+        delay = t0 - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        ids.append(server.submit(gid, w.holograms))
+    for rid in ids:
+        server.wait(rid)
+    st = server.stats()
+    out["served_poisson"] = {"rate_per_s": round(rate, 3), "recon_s": round(st["recon_per_s"], 3),
+                             "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
+                             "requests": served_requests}
+    log(f"served poisson: {out['served_poisson']}")
+    out["served_vs_plain"] = round(out["served_burst"]["recon_s"] / out["plain"]["recon_s"], 3)
+    out["served_equals_pipeline_bitwise"] = bool(np.array_equal(ri_served, ri_pipe))
+    nm = g.n_medium
+    out["plain_vs_served_contrast_rel_l2"] = float(
+        np.sqrt(np.sum((ri_plain.astype(np.float64) - ri_served) ** 2) /
+                np.sum((ri_served.astype(np.float64) - nm) ** 2)))
+    return out, ri_served
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,6 +193,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true",
                     help="skip the oracle run (CPU baseline and full-size parity)")
+    ap.add_argument("--no-serving", action="store_true",
+                    help="skip the served-versus-plain comparison (host-memory requests)")
     ap.add_argument("--fgp-traffic", action="store_true",
                     help="run only 5 FGP steps at 256^3 and a 256 MiB device copy, for the "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE runs that calibrate HBM traffic")
@@ -196,11 +291,19 @@ def main() -> int:
             "volume": [N, N, N],
             "solver": f"fista-tv {ITERATIONS}x{INNER}",
         },
-        "latency_ms": {"plain": {"p50": round(float(p50), 3), "p99": round(float(p99), 3)},
-                       "served": None},
+        "latency_ms": {"device_resident": {"p50": round(float(p50), 3),
+                                           "p99": round(float(p99), 3)}},
         "stage_ms": stages,
         "roofline": roofline,
     }
+
+    if not args.no_serving:
+        serving, _ = serving_comparison(qh, w)
+        result["serving"] = serving
+        result["latency_ms"]["plain"] = {"p50": serving["plain"]["p50_ms"],
+                                         "p99": serving["plain"]["p99_ms"]}
+        result["latency_ms"]["served"] = {"p50": serving["served_poisson"]["p50_ms"],
+                                          "p99": serving["served_poisson"]["p99_ms"]}
 
     if not args.no_cpu_baseline:
         result["cpu_baseline"], want = cpu_baseline(w)

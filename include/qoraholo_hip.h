@@ -201,6 +201,45 @@ int qh_hip_pipeline_output_shape(const qh_pipeline* p, int* nz, int* ny, int* nx
 /* Records FGP kernel intervals in timings (a few events per outer iteration). */
 int qh_hip_pipeline_time_fgp(qh_pipeline* p, int on);
 
+/* ---- Served loop (no counterpart in the reference, which serves one request per Pipeline
+ * call): requests of several geometries run concurrently in device-resident slots admitted
+ * against an HBM budget. ---- */
+
+typedef struct qh_server qh_server;
+
+typedef struct {
+  int64_t submitted;
+  int64_t completed;
+  int64_t failed;
+  int64_t queued;
+  double p50_ms; /* latency from submit to result, completed requests */
+  double p99_ms;
+  double mean_ms;
+  double recon_per_s; /* completed / (last completion - first submission) */
+  size_t device_bytes;
+  size_t budget_bytes;
+  int slots;
+  int busy_slots;
+} qh_server_stats;
+
+/* budget_bytes 0: 90% of the free device memory. max_slots: requests in flight at once. */
+int qh_hip_server_create(size_t budget_bytes, int max_slots, qh_server** out);
+/* Finishes the requests on the device, drops the queued ones, frees everything. */
+void qh_hip_server_destroy(qh_server* s);
+/* A geometry (the options of a Pipeline) and its background holograms (count, N, N). */
+int qh_hip_server_add_geometry(qh_server* s, const qh_pipeline_options* options,
+                               const float* backgrounds, int* geometry);
+/* Queues holograms (count, N, N) of a geometry; the result (nz, n, n) goes to ri_out. Both
+ * host buffers must stay valid until qh_hip_server_wait returns for the id. Thread-safe. */
+int qh_hip_server_submit(qh_server* s, int geometry, const float* holograms, float* ri_out,
+                         int64_t* id);
+/* Blocks until the request is done and returns its status (QH_OK or the error it failed with,
+ * message in qh_hip_last_error); one failed request does not affect the others. */
+int qh_hip_server_wait(qh_server* s, int64_t id, double* filled, double* latency_ms);
+int qh_hip_server_poll(qh_server* s, int64_t id, int* done);
+int qh_hip_server_stats(qh_server* s, qh_server_stats* out);
+int qh_hip_server_reset_stats(qh_server* s);
+
 /* ---- transform and TV test hooks (no counterpart in the reference's ABI: it used cuFFT) ---- */
 
 /* In-place complex FFT over the axes in `axes` (bit 0: z, 1: y, 2: x) of (d0, d1, d2). */

@@ -26,6 +26,7 @@ __all__ = [
     "LogicError",
     "OutOfDeviceMemory",
     "Pipeline",
+    "Server",
     "Sideband",
     "Sphere",
     "circular_scan",
@@ -292,6 +293,28 @@ def kspace(fields, illuminations, geometry: Geometry, nz: int = 0):
     return pot, counts, filled.value
 
 
+def _pipeline_options(count, hologram_size, center, illuminations, geometry, radius, output_size,
+                      nz, iterations, solver, gp_transform, tv_weight, tv_inner, mapping,
+                      upload_chunks):
+    """PipelineOptions (include/qoraholo/pipeline.hpp:15-25) as the C-ABI struct, and the
+    illumination array it points to (keep both alive together)."""
+    if center is None:
+        raise ValueError("Pipeline: field.center (the carrier) is required")
+    if count < 1:
+        raise ValueError("Pipeline: count must be positive")
+    il = _illum(illuminations)
+    if il.shape[0] != count:
+        raise ValueError("Pipeline: need one illumination per hologram")
+    illum_c = (_native.Illumination_t * max(1, il.shape[0]))()
+    for a in range(il.shape[0]):
+        illum_c[a] = _native.Illumination_t(il[a, 0], il[a, 1])
+    plan = _crop(hologram_size, center, radius, output_size)
+    opts = _options(nz, iterations, solver, gp_transform, tv_weight, tv_inner, mapping)
+    o = _native.PipelineOptions_t(int(count), int(hologram_size), plan, illum_c, geometry._c(),
+                                  opts, int(upload_chunks))
+    return o, illum_c
+
+
 class Pipeline:
     """Holograms to refractive index with every buffer allocated once on the device.
 
@@ -308,20 +331,10 @@ class Pipeline:
                  backend: str = "hip", upload_chunks: int = 1):
         _backend(backend)
         self._handle = None
-        if center is None:
-            raise ValueError("Pipeline: field.center (the carrier) is required")
-        il = _illum(illuminations)
-        self._illum_c = (_native.Illumination_t * max(1, il.shape[0]))()
-        for a in range(il.shape[0]):
-            self._illum_c[a] = _native.Illumination_t(il[a, 0], il[a, 1])
-        if count < 1:
-            raise ValueError("Pipeline: count must be positive")
-        plan = _crop(hologram_size, center, radius, output_size)
-        opts = _options(nz, iterations, solver, gp_transform, tv_weight, tv_inner, mapping)
-        if il.shape[0] != count:
-            raise ValueError("Pipeline: need one illumination per hologram")
-        o = _native.PipelineOptions_t(int(count), int(hologram_size), plan, self._illum_c,
-                                      geometry._c(), opts, int(upload_chunks))
+        o, self._illum_c = _pipeline_options(count, hologram_size, center, illuminations, geometry,
+                                             radius, output_size, nz, iterations, solver,
+                                             gp_transform, tv_weight, tv_inner, mapping,
+                                             upload_chunks)
         handle = ctypes.c_void_p()
         _native.check(_native.lib().qh_hip_pipeline_create(ctypes.byref(o), ctypes.byref(handle)))
         self._handle = handle
@@ -407,3 +420,85 @@ class Pipeline:
     @property
     def device_bytes(self) -> int:
         return int(_native.lib().qh_hip_pipeline_device_bytes(self._handle))
+
+
+class Server:
+    """Served reconstructions: requests of several geometries admitted against an HBM budget.
+
+    ``add_geometry`` registers the options of a Pipeline with its background holograms and
+    returns a geometry id; ``submit`` queues sample holograms of a geometry (thread-safe) and
+    returns a request id; ``wait`` returns (ri, filled, latency_ms). Requests that arrive while
+    others run are admitted into free device-resident slots (created while ``budget_bytes``
+    allows, at most ``max_slots`` in flight) and overlap on the device; each result is the one
+    the request gets alone.
+    """
+
+    def __init__(self, budget_bytes: int = 0, max_slots: int = 4):
+        self._handle = None
+        handle = ctypes.c_void_p()
+        _native.check(_native.lib().qh_hip_server_create(int(budget_bytes), int(max_slots),
+                                                         ctypes.byref(handle)))
+        self._handle = handle
+        self._geometries = {}
+        self._pending = {}
+
+    def __del__(self):
+        if getattr(self, "_handle", None):
+            _native.lib().qh_hip_server_destroy(self._handle)
+            self._handle = None
+
+    def add_geometry(self, count: int, hologram_size: int, center, illuminations,
+                     geometry: Geometry, backgrounds, radius: float = 0.0, output_size: int = 0,
+                     nz: int = 0, iterations: int = 0, solver: str = "gp",
+                     gp_transform: str = "r2c", tv_weight: float = 1e-2, tv_inner: int = 10,
+                     mapping: str = "gather") -> int:
+        o, illum_c = _pipeline_options(count, hologram_size, center, illuminations, geometry,
+                                       radius, output_size, nz, iterations, solver, gp_transform,
+                                       tv_weight, tv_inner, mapping, 1)
+        b = np.ascontiguousarray(backgrounds, dtype=np.float32)
+        if b.shape != (count, hologram_size, hologram_size):
+            raise ValueError("Server: backgrounds must be (count, N, N)")
+        gid = ctypes.c_int()
+        _native.check(_native.lib().qh_hip_server_add_geometry(self._handle, ctypes.byref(o),
+                                                               _native.ptr(b), ctypes.byref(gid)))
+        m = o.crop.out
+        self._geometries[gid.value] = ((count, hologram_size, hologram_size),
+                                       (nz if nz > 0 else m, m, m))
+        return gid.value
+
+    def submit(self, geometry: int, holograms) -> int:
+        if geometry not in self._geometries:
+            raise ValueError("Server::submit: unknown geometry")
+        shape_in, shape_out = self._geometries[geometry]
+        h = np.ascontiguousarray(holograms, dtype=np.float32)
+        if h.shape != shape_in:
+            raise ValueError(f"Server::submit: holograms must be {shape_in}")
+        out = np.empty(shape_out, np.float32)
+        rid = ctypes.c_int64()
+        _native.check(_native.lib().qh_hip_server_submit(self._handle, int(geometry),
+                                                         _native.ptr(h), _native.ptr(out),
+                                                         ctypes.byref(rid)))
+        self._pending[rid.value] = (h, out)
+        return rid.value
+
+    def poll(self, request: int) -> bool:
+        done = ctypes.c_int()
+        _native.check(_native.lib().qh_hip_server_poll(self._handle, int(request), ctypes.byref(done)))
+        return bool(done.value)
+
+    def wait(self, request: int):
+        filled = ctypes.c_double()
+        latency = ctypes.c_double()
+        rc = _native.lib().qh_hip_server_wait(self._handle, int(request), ctypes.byref(filled),
+                                              ctypes.byref(latency))
+        _, out = self._pending.pop(int(request), (None, None))
+        _native.check(rc)
+        return out, filled.value, latency.value
+
+    def stats(self) -> dict:
+        s = _native.ServerStats_t()
+        _native.check(_native.lib().qh_hip_server_stats(self._handle, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in s._fields_}
+
+    def reset_stats(self) -> None:
+        _native.check(_native.lib().qh_hip_server_reset_stats(self._handle))

@@ -57,6 +57,14 @@ class PipelineTimings_t(ctypes.Structure):
         "download", "total", "fgp_ms")] + [("fgp_steps", c_int)]
 
 
+class ServerStats_t(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_int64), ("completed", ctypes.c_int64),
+                ("failed", ctypes.c_int64), ("queued", ctypes.c_int64), ("p50_ms", c_double),
+                ("p99_ms", c_double), ("mean_ms", c_double), ("recon_per_s", c_double),
+                ("device_bytes", c_size_t), ("budget_bytes", c_size_t), ("slots", c_int),
+                ("busy_slots", c_int)]
+
+
 class PipelineOptions_t(ctypes.Structure):
     _fields_ = [("count", c_int), ("hologram_size", c_int), ("crop", CropPlan_t),
                 ("illuminations", POINTER(Illumination_t)), ("geometry", Geometry_t),
@@ -91,6 +99,15 @@ _SIGNATURES = {
     "qh_hip_pipeline_device_bytes": (c_size_t, [c_void_p]),
     "qh_hip_pipeline_output_shape": (c_int, [c_void_p, POINTER(c_int), POINTER(c_int), POINTER(c_int)]),
     "qh_hip_pipeline_time_fgp": (c_int, [c_void_p, c_int]),
+    "qh_hip_server_create": (c_int, [c_size_t, c_int, POINTER(c_void_p)]),
+    "qh_hip_server_destroy": (None, [c_void_p]),
+    "qh_hip_server_add_geometry": (c_int, [c_void_p, POINTER(PipelineOptions_t), c_void_p,
+                                           POINTER(c_int)]),
+    "qh_hip_server_submit": (c_int, [c_void_p, c_int, c_void_p, c_void_p, POINTER(ctypes.c_int64)]),
+    "qh_hip_server_wait": (c_int, [c_void_p, ctypes.c_int64, POINTER(c_double), POINTER(c_double)]),
+    "qh_hip_server_poll": (c_int, [c_void_p, ctypes.c_int64, POINTER(c_int)]),
+    "qh_hip_server_stats": (c_int, [c_void_p, POINTER(ServerStats_t)]),
+    "qh_hip_server_reset_stats": (c_int, [c_void_p]),
     "qh_hip_fft_c2c": (c_int, [c_void_p, c_int, c_int, c_int, c_uint, c_int, c_float]),
     "qh_hip_rfft3": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p]),
     "qh_hip_irfft3": (c_int, [c_void_p, c_int, c_int, c_int, c_float, c_void_p]),

@@ -14,10 +14,15 @@
 #include "fft.hpp"
 #include "pipeline.hpp"
 #include "runtime.hpp"
+#include "server.hpp"
 #include "stages.hpp"
 
 struct qh_pipeline {
   std::unique_ptr<qh::Pipeline> impl;
+};
+
+struct qh_server {
+  std::unique_ptr<qh::Server> impl;
 };
 
 namespace qh {
@@ -112,6 +117,19 @@ void check_plan(const qh_crop_plan& p, int n) {
 }
 
 }  // namespace
+
+// Checks of the Pipeline constructor (src/pipeline.cpp:72-91) and of odt_grid.
+void check_pipeline_options(const qh_pipeline_options& o) {
+  const int n = o.hologram_size;
+  require(o.count >= 1, "Pipeline: count must be positive");
+  require(n >= 4 && n % 2 == 0, "Pipeline: hologram_size must be even");
+  require(o.upload_chunks >= 1 && o.upload_chunks <= o.count,
+          "Pipeline: upload_chunks must be between 1 and count");
+  check_plan(o.crop, n);
+  require_fft(n, "Pipeline");
+  require_fft(o.crop.out, "Pipeline");
+  checked_grid(o.crop.out, o.count, o.illuminations, o.geometry, o.odt, "Pipeline");
+}
 
 OdtGrid grid_from(const qh_pipeline_options& o) {
   return checked_grid(o.crop.out, o.count, o.illuminations, o.geometry, o.odt, "Pipeline");
@@ -355,15 +373,7 @@ int qh_hip_pipeline_create(const qh_pipeline_options* o, qh_pipeline** out) {
   return guarded([&] {
     require(o && out, "Pipeline: null argument");
     *out = nullptr;
-    const int n = o->hologram_size;
-    require(o->count >= 1, "Pipeline: count must be positive");
-    require(n >= 4 && n % 2 == 0, "Pipeline: hologram_size must be even");
-    require(o->upload_chunks >= 1 && o->upload_chunks <= o->count,
-            "Pipeline: upload_chunks must be between 1 and count");
-    check_plan(o->crop, n);
-    require_fft(n, "Pipeline");
-    require_fft(o->crop.out, "Pipeline");
-    checked_grid(o->crop.out, o->count, o->illuminations, o->geometry, o->odt, "Pipeline");
+    check_pipeline_options(*o);
     require_device();
     const auto t0 = std::chrono::steady_clock::now();
     std::size_t free_before = 0, free_after = 0, total_mem = 0;
@@ -433,6 +443,69 @@ int qh_hip_pipeline_time_fgp(qh_pipeline* p, int on) {
   return guarded([&] {
     require(p != nullptr, "Pipeline: null handle");
     p->impl->time_fgp(on != 0);
+  });
+}
+
+int qh_hip_server_create(size_t budget_bytes, int max_slots, qh_server** out) {
+  return guarded([&] {
+    require(out != nullptr, "Server: null argument");
+    *out = nullptr;
+    require(max_slots >= 1, "Server: max_slots must be positive");
+    require_device();
+    auto s = std::make_unique<qh_server>();
+    s->impl = std::make_unique<Server>(budget_bytes, max_slots);
+    *out = s.release();
+  });
+}
+
+void qh_hip_server_destroy(qh_server* s) { delete s; }
+
+int qh_hip_server_add_geometry(qh_server* s, const qh_pipeline_options* options,
+                               const float* backgrounds, int* geometry) {
+  return guarded([&] {
+    require(s && options && backgrounds && geometry, "Server: null argument");
+    *geometry = s->impl->add_geometry(*options, backgrounds);
+  });
+}
+
+int qh_hip_server_submit(qh_server* s, int geometry, const float* holograms, float* ri_out,
+                         int64_t* id) {
+  return guarded([&] {
+    require(s && holograms && ri_out && id, "Server: null argument");
+    *id = s->impl->submit(geometry, holograms, ri_out);
+  });
+}
+
+int qh_hip_server_wait(qh_server* s, int64_t id, double* filled, double* latency_ms) {
+  int status = QH_OK;
+  const int rc = guarded([&] {
+    require(s != nullptr, "Server: null handle");
+    std::string error;
+    status = s->impl->wait(id, filled, latency_ms, &error);
+    if (status != QH_OK) throw std::runtime_error(error);
+  });
+  if (rc != QH_OK && status != QH_OK) return status;
+  return rc;
+}
+
+int qh_hip_server_poll(qh_server* s, int64_t id, int* done) {
+  return guarded([&] {
+    require(s && done, "Server: null argument");
+    *done = s->impl->done(id) ? 1 : 0;
+  });
+}
+
+int qh_hip_server_stats(qh_server* s, qh_server_stats* out) {
+  return guarded([&] {
+    require(s && out, "Server: null argument");
+    *out = s->impl->stats();
+  });
+}
+
+int qh_hip_server_reset_stats(qh_server* s) {
+  return guarded([&] {
+    require(s != nullptr, "Server: null handle");
+    s->impl->reset_stats();
   });
 }
 

@@ -1,0 +1,294 @@
+// Served reconstruction loop (see server.hpp).
+#include <algorithm>
+#include <cstring>
+
+#include "server.hpp"
+
+namespace qh {
+
+void check_pipeline_options(const qh_pipeline_options& o);
+
+namespace {
+
+double ms_between(std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
+  return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+std::size_t free_device_bytes() {
+  std::size_t free_b = 0, total_b = 0;
+  check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+  return free_b;
+}
+
+}  // namespace
+
+Server::Server(std::size_t budget_bytes, int max_slots)
+    : budget_(budget_bytes), max_slots_(std::max(1, max_slots)) {
+  if (budget_ == 0) budget_ = free_device_bytes() / 10 * 9;
+  dispatcher_ = std::thread([this] { dispatch_loop(); });
+}
+
+Server::~Server() {
+  {
+    std::lock_guard<std::mutex> lock(mu_);
+    stop_ = true;
+    for (auto& s : slots_) s->stop = true;
+  }
+  queue_cv_.notify_all();
+  for (auto& s : slots_) s->cv.notify_all();
+  if (dispatcher_.joinable()) dispatcher_.join();
+  for (auto* list : {&slots_, &retired_}) {
+    for (auto& s : *list) {
+      s->cv.notify_all();
+      if (s->thread.joinable()) s->thread.join();
+    }
+  }
+}
+
+void Server::reset_stats() {
+  std::lock_guard<std::mutex> lock(mu_);
+  latencies_.clear();
+  submitted_ = completed_ = failed_ = 0;
+}
+
+int Server::add_geometry(const qh_pipeline_options& o, const float* backgrounds) {
+  check_pipeline_options(o);
+  require(backgrounds != nullptr, "Server: background holograms are required");
+  auto g = std::make_unique<Geometry>();
+  g->options = o;
+  g->illuminations.assign(o.illuminations, o.illuminations + o.count);
+  g->options.illuminations = g->illuminations.data();
+  const std::size_t image = static_cast<std::size_t>(o.hologram_size) * o.hologram_size;
+  g->backgrounds.assign(backgrounds, backgrounds + image * o.count);
+  // The first slot is built now: it measures the footprint and fails early when it does not fit.
+  const std::size_t before = free_device_bytes();
+  auto slot = std::make_unique<Slot>();
+  slot->pipeline = std::make_unique<Pipeline>(g->options);
+  slot->pipeline->set_background(g->backgrounds.data());
+  check(hipDeviceSynchronize(), "Server::add_geometry");
+  const std::size_t after = free_device_bytes();
+  slot->bytes = before > after ? before - after : 0;
+  g->slot_bytes = std::max<std::size_t>(slot->bytes, 1);
+  g->nz = slot->pipeline->nz();
+  g->n = slot->pipeline->n();
+  std::lock_guard<std::mutex> lock(mu_);
+  if (used_bytes_ + slot->bytes > budget_) {
+    throw OutOfMemory("Server: one slot of this geometry needs " + std::to_string(slot->bytes) +
+                      " bytes, over the HBM budget");
+  }
+  const int id = static_cast<int>(geometries_.size());
+  slot->geometry = id;
+  g->slots = 1;
+  used_bytes_ += slot->bytes;
+  Slot* raw = slot.get();
+  slots_.push_back(std::move(slot));
+  raw->thread = std::thread([this, raw] { slot_loop(raw); });
+  geometries_.push_back(std::move(g));
+  return id;
+}
+
+std::int64_t Server::submit(int geometry, const float* holograms, float* ri_out) {
+  require(holograms != nullptr && ri_out != nullptr, "Server::submit: null buffer");
+  auto r = std::make_shared<Request>();
+  r->holograms = holograms;
+  r->out = ri_out;
+  r->arrival = std::chrono::steady_clock::now();
+  {
+    std::lock_guard<std::mutex> lock(mu_);
+    require(geometry >= 0 && geometry < static_cast<int>(geometries_.size()),
+            "Server::submit: unknown geometry");
+    r->geometry = geometry;
+    r->id = next_id_++;
+    if (submitted_ == 0) first_arrival_ = r->arrival;
+    ++submitted_;
+    requests_[r->id] = r;
+    queue_.push_back(r);
+  }
+  queue_cv_.notify_all();
+  return r->id;
+}
+
+// Called with mu_ held: an idle slot of the request's geometry, or a new slot when the budget
+// and the slot limit allow one (evicting idle slots of other geometries if needed).
+bool Server::try_admit(const std::shared_ptr<Request>& r) {
+  for (auto& s : slots_) {
+    if (!s->busy && !s->stop && s->geometry == r->geometry && s->pipeline) {
+      s->busy = true;
+      s->work = r;
+      s->cv.notify_all();
+      return true;
+    }
+  }
+  Geometry& g = *geometries_[r->geometry];
+  const auto live = [&] {
+    int n = 0;
+    for (auto& s : slots_) n += s->pipeline ? 1 : 0;
+    return n;
+  };
+  while ((used_bytes_ + g.slot_bytes > budget_ || live() >= max_slots_)) {
+    // Evict one idle slot of another geometry; stop when none is idle.
+    auto it = std::find_if(slots_.begin(), slots_.end(), [&](const std::unique_ptr<Slot>& s) {
+      return s->pipeline && !s->busy && s->geometry != r->geometry;
+    });
+    if (it == slots_.end()) return false;
+    // The victim is idle: free its device memory now; its thread leaves its wait once the lock
+    // is released and is joined with the retired slots.
+    Slot* victim = it->get();
+    victim->pipeline.reset();
+    victim->stop = true;
+    victim->cv.notify_all();
+    used_bytes_ -= std::min(used_bytes_, victim->bytes);
+    --geometries_[victim->geometry]->slots;
+    retired_.push_back(std::move(*it));
+    slots_.erase(it);
+  }
+  auto slot = std::make_unique<Slot>();
+  slot->geometry = r->geometry;
+  slot->pipeline = std::make_unique<Pipeline>(g.options);
+  slot->pipeline->set_background(g.backgrounds.data());
+  slot->bytes = g.slot_bytes;
+  used_bytes_ += slot->bytes;
+  ++g.slots;
+  slot->busy = true;
+  slot->work = r;
+  Slot* raw = slot.get();
+  slots_.push_back(std::move(slot));
+  raw->thread = std::thread([this, raw] { slot_loop(raw); });
+  return true;
+}
+
+void Server::dispatch_loop() {
+  std::unique_lock<std::mutex> lock(mu_);
+  while (!stop_) {
+    bool progress = false;
+    for (auto it = queue_.begin(); it != queue_.end();) {
+      bool admitted = false;
+      try {
+        admitted = try_admit(*it);
+      } catch (const std::exception& e) {
+        // A slot that cannot be built fails this request only.
+        auto& r = **it;
+        r.finished = true;
+        r.status = dynamic_cast<const OutOfMemory*>(&e) ? QH_ERR_OUT_OF_MEMORY : QH_ERR_RUNTIME;
+        r.error = e.what();
+        ++failed_;
+        done_cv_.notify_all();
+        admitted = true;
+      }
+      if (admitted) {
+        it = queue_.erase(it);
+        progress = true;
+      } else {
+        ++it;
+      }
+    }
+    if (!progress) queue_cv_.wait(lock);
+  }
+}
+
+void Server::slot_loop(Slot* slot) {
+  std::unique_lock<std::mutex> lock(mu_);
+  while (true) {
+    slot->cv.wait(lock, [&] { return slot->stop || slot->work; });
+    if (slot->stop && !slot->work) return;
+    std::shared_ptr<Request> r = slot->work;
+    const Geometry& g = *geometries_[r->geometry];
+    const std::size_t in_count = static_cast<std::size_t>(g.options.count) *
+                                 g.options.hologram_size * g.options.hologram_size;
+    const std::size_t out_count = static_cast<std::size_t>(g.nz) * g.n * g.n;
+    lock.unlock();
+    int status = QH_OK;
+    std::string error;
+    double filled = 0;
+    try {
+      std::memcpy(slot->pipeline->input(), r->holograms, in_count * sizeof(float));
+      slot->pipeline->process();
+      std::memcpy(r->out, slot->pipeline->output(), out_count * sizeof(float));
+      filled = slot->pipeline->filled();
+    } catch (const InvalidArgument& e) {
+      status = QH_ERR_INVALID_ARGUMENT;
+      error = e.what();
+    } catch (const OutOfMemory& e) {
+      status = QH_ERR_OUT_OF_MEMORY;
+      error = e.what();
+    } catch (const std::exception& e) {
+      status = QH_ERR_RUNTIME;
+      error = e.what();
+    }
+    const auto now = std::chrono::steady_clock::now();
+    lock.lock();
+    r->finished = true;
+    r->status = status;
+    r->error = error;
+    r->filled = filled;
+    r->latency_ms = ms_between(r->arrival, now);
+    if (status == QH_OK) {
+      ++completed_;
+      latencies_.push_back(r->latency_ms);
+      last_done_ = now;
+    } else {
+      ++failed_;
+    }
+    slot->work.reset();
+    slot->busy = false;
+    done_cv_.notify_all();
+    queue_cv_.notify_all();
+  }
+}
+
+int Server::wait(std::int64_t id, double* filled, double* latency_ms, std::string* error) {
+  std::unique_lock<std::mutex> lock(mu_);
+  auto it = requests_.find(id);
+  require(it != requests_.end(), "Server::wait: unknown request id");
+  std::shared_ptr<Request> r = it->second;
+  done_cv_.wait(lock, [&] { return r->finished; });
+  requests_.erase(id);
+  if (filled) *filled = r->filled;
+  if (latency_ms) *latency_ms = r->latency_ms;
+  if (error) *error = r->error;
+  return r->status;
+}
+
+bool Server::done(std::int64_t id) {
+  std::lock_guard<std::mutex> lock(mu_);
+  auto it = requests_.find(id);
+  require(it != requests_.end(), "Server::poll: unknown request id");
+  return it->second->finished;
+}
+
+qh_server_stats Server::stats() {
+  std::lock_guard<std::mutex> lock(mu_);
+  qh_server_stats s{};
+  s.submitted = submitted_;
+  s.completed = completed_;
+  s.failed = failed_;
+  s.queued = static_cast<std::int64_t>(queue_.size());
+  std::vector<double> l = latencies_;
+  std::sort(l.begin(), l.end());
+  const auto pct = [&](double q) {
+    if (l.empty()) return 0.0;
+    const double pos = q * static_cast<double>(l.size() - 1);
+    const std::size_t i = static_cast<std::size_t>(pos);
+    const double f = pos - static_cast<double>(i);
+    return i + 1 < l.size() ? l[i] * (1.0 - f) + l[i + 1] * f : l[i];
+  };
+  s.p50_ms = pct(0.50);
+  s.p99_ms = pct(0.99);
+  double sum = 0;
+  for (double v : l) sum += v;
+  s.mean_ms = l.empty() ? 0.0 : sum / static_cast<double>(l.size());
+  const double span = completed_ > 0 ? ms_between(first_arrival_, last_done_) : 0.0;
+  s.recon_per_s = span > 0 ? 1000.0 * static_cast<double>(completed_) / span : 0.0;
+  s.device_bytes = used_bytes_;
+  s.budget_bytes = budget_;
+  int live = 0, busy = 0;
+  for (auto& sl : slots_) {
+    live += sl->pipeline ? 1 : 0;
+    busy += sl->busy ? 1 : 0;
+  }
+  s.slots = live;
+  s.busy_slots = busy;
+  return s;
+}
+
+}  // namespace qh

@@ -1,0 +1,97 @@
+// Served reconstruction loop: requests of different geometries (hologram count and size,
+// volume size, solver) arrive on host threads; each geometry keeps its background fields, plans
+// and solver state resident in device-resident slots; a dispatcher admits queued requests into
+// free slots, creating slots while the HBM budget allows, so requests that arrive while others
+// are on the device start as soon as a slot frees up and run concurrently on their own streams.
+#pragma once
+
+#include <chrono>
+#include <condition_variable>
+#include <cstdint>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/qoraholo_hip.h"
+#include "pipeline.hpp"
+
+namespace qh {
+
+class Server {
+ public:
+  Server(std::size_t budget_bytes, int max_slots);
+  ~Server();
+  Server(const Server&) = delete;
+  Server& operator=(const Server&) = delete;
+
+  // Registers a geometry with its background holograms (count, N, N); returns its id.
+  int add_geometry(const qh_pipeline_options& o, const float* backgrounds);
+  // Queues a request; holograms (count, N, N) and ri_out (nz, n, n) are host memory that must
+  // stay valid until wait() returns for this id.
+  std::int64_t submit(int geometry, const float* holograms, float* ri_out);
+  // Blocks until the request is done; returns its status and message.
+  int wait(std::int64_t id, double* filled, double* latency_ms, std::string* error);
+  bool done(std::int64_t id);
+  qh_server_stats stats();
+  void reset_stats();
+
+ private:
+  struct Geometry {
+    qh_pipeline_options options;
+    std::vector<qh_illumination> illuminations;
+    std::vector<float> backgrounds;
+    std::size_t slot_bytes = 0;  // device bytes of one slot, measured on the first
+    int slots = 0;
+    int nz = 0, n = 0;
+  };
+  struct Request {
+    std::int64_t id;
+    int geometry;
+    const float* holograms;
+    float* out;
+    std::chrono::steady_clock::time_point arrival;
+    bool finished = false;
+    int status = QH_OK;
+    std::string error;
+    double filled = 0;
+    double latency_ms = 0;
+  };
+  struct Slot {
+    int geometry;
+    std::unique_ptr<Pipeline> pipeline;
+    std::size_t bytes = 0;
+    std::shared_ptr<Request> work;
+    bool busy = false;
+    bool stop = false;
+    std::condition_variable cv;
+    std::thread thread;
+  };
+
+  void dispatch_loop();
+  void slot_loop(Slot* slot);
+  bool try_admit(const std::shared_ptr<Request>& r);
+
+  std::size_t budget_;
+  int max_slots_;
+  std::mutex mu_;
+  std::condition_variable queue_cv_;
+  std::condition_variable done_cv_;
+  std::deque<std::shared_ptr<Request>> queue_;
+  std::map<std::int64_t, std::shared_ptr<Request>> requests_;
+  std::vector<std::unique_ptr<Geometry>> geometries_;
+  std::vector<std::unique_ptr<Slot>> slots_;
+  std::vector<std::unique_ptr<Slot>> retired_;
+  std::size_t used_bytes_ = 0;
+  std::int64_t next_id_ = 1;
+  bool stop_ = false;
+  std::vector<double> latencies_;
+  std::int64_t submitted_ = 0, completed_ = 0, failed_ = 0;
+  std::chrono::steady_clock::time_point first_arrival_, last_done_;
+  std::thread dispatcher_;
+};
+
+}  // namespace qh
