@@ -7,6 +7,10 @@ compute runs in ``libqoraholo_hip.so`` (hand-written gfx950 kernels behind the C
 ``include/qoraholo_hip.h``). ``field_ratio`` and ``find_sideband`` are host functions, as in the
 reference. Only ``backend="hip"`` is served here; the reference's own CPU backend is not part
 of this package.
+
+With PyTorch in the same process, import torch first: torch carries its own HIP runtime under
+the same soname, and the library then shares it; loaded the other way round the process would
+hold two runtimes.
 """
 
 from __future__ import annotations

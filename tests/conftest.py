@@ -5,6 +5,14 @@ from pathlib import Path
 import numpy as np
 import pytest
 
+# One HIP runtime per process: torch ships its own libamdhip64 (same soname as ROCm's). Loaded
+# first, it is the runtime libqoraholo_hip.so binds to; loaded after the library, it would be a
+# second runtime that cannot open the device. The rocFFT cross-checks use torch, so it goes first.
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 ROOT = Path(__file__).resolve().parent.parent
 if str(ROOT) not in sys.path:
     sys.path.insert(0, str(ROOT))
