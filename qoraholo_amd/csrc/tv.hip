@@ -131,6 +131,34 @@ __device__ __forceinline__ void primal_plane(const TvArgs& a, int zp, int y0, in
   }
 }
 
+// Data of one plane that the march loads ahead: r on the strip rows and the halo row below,
+// the y component of r on the row above, b on the same rows, and p on the strip rows of the
+// plane before it (the plane whose dual step runs while this plane's primal is formed).
+template <int VX, int TY>
+struct PlaneLoad {
+  float r[3][TY + 1][VX];
+  float ry_top[VX];
+  float b[TY + 1][VX];
+  float p[3][TY][VX];
+};
+
+template <int VX, int TY>
+__device__ __forceinline__ void load_plane(const TvArgs& a, int z, int y0, int xb, bool vec,
+                                           PlaneLoad<VX, TY>& L) {
+#pragma unroll
+  for (int j = 0; j <= TY; ++j) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) load_row<VX>(a.r_in + c * a.vol, a, z, y0 + j, xb, vec, L.r[c][j]);
+    load_row<VX>(a.b, a, z, y0 + j, xb, vec, L.b[j]);
+  }
+  load_row<VX>(a.r_in + a.vol, a, z, y0 - 1, xb, vec, L.ry_top);
+#pragma unroll
+  for (int j = 0; j < TY; ++j) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) load_row<VX>(a.p_in + c * a.vol, a, z - 1, y0 + j, xb, vec, L.p[c][j]);
+  }
+}
+
 template <int VX, int TY>
 __global__ void __launch_bounds__(64) k_fgp_step(TvArgs a, int chunks) {
   const int t = xcd_tile();
@@ -140,40 +168,32 @@ __global__ void __launch_bounds__(64) k_fgp_step(TvArgs a, int chunks) {
   const int xb = static_cast<int>(threadIdx.x) * VX;
   const bool vec = (a.nx % 4) == 0;
 
-  float r[3][TY + 1][VX];   // r at plane z, rows y0..y0+TY
-  float zc[TY + 1][VX];     // primal at plane z
-  float rn[3][TY + 1][VX];  // r at plane z + 1
+  float r[3][TY + 1][VX];  // r at plane z, rows y0..y0+TY
+  float zc[TY + 1][VX];    // primal at plane z
   float zn[TY + 1][VX];
-  float bb[TY + 1][VX];
-  float ry_top[VX];
-  float rz_below[TY + 1][VX];
 
   // Warm-up: the primal on plane z0 needs r_z on plane z0 - 1.
-#pragma unroll
-  for (int j = 0; j <= TY; ++j) {
-    load_row<VX>(a.r_in, a, z0 - 1, y0 + j, xb, vec, rz_below[j]);
-#pragma unroll
-    for (int c = 0; c < 3; ++c) load_row<VX>(a.r_in + c * a.vol, a, z0, y0 + j, xb, vec, r[c][j]);
-    load_row<VX>(a.b, a, z0, y0 + j, xb, vec, bb[j]);
-  }
-  load_row<VX>(a.r_in + a.vol, a, z0, y0 - 1, xb, vec, ry_top);
-  primal_plane<VX, TY>(a, z0, y0, xb, r, ry_top, rz_below, bb, zc);
-
-  for (int z = z0; z < z1; ++z) {
-    float pv[3][TY][VX];
-#pragma unroll
-    for (int j = 0; j < TY; ++j) {
-#pragma unroll
-      for (int c = 0; c < 3; ++c) load_row<VX>(a.p_in + c * a.vol, a, z, y0 + j, xb, vec, pv[c][j]);
-    }
+  {
+    float rz_below[TY + 1][VX];
+    float bb[TY + 1][VX];
+    float ry_top[VX];
 #pragma unroll
     for (int j = 0; j <= TY; ++j) {
+      load_row<VX>(a.r_in, a, z0 - 1, y0 + j, xb, vec, rz_below[j]);
 #pragma unroll
-      for (int c = 0; c < 3; ++c) load_row<VX>(a.r_in + c * a.vol, a, z + 1, y0 + j, xb, vec, rn[c][j]);
-      load_row<VX>(a.b, a, z + 1, y0 + j, xb, vec, bb[j]);
+      for (int c = 0; c < 3; ++c) load_row<VX>(a.r_in + c * a.vol, a, z0, y0 + j, xb, vec, r[c][j]);
+      load_row<VX>(a.b, a, z0, y0 + j, xb, vec, bb[j]);
     }
-    load_row<VX>(a.r_in + a.vol, a, z + 1, y0 - 1, xb, vec, ry_top);
-    primal_plane<VX, TY>(a, z + 1, y0, xb, rn, ry_top, r[0], bb, zn);
+    load_row<VX>(a.r_in + a.vol, a, z0, y0 - 1, xb, vec, ry_top);
+    primal_plane<VX, TY>(a, z0, y0, xb, r, ry_top, rz_below, bb, zc);
+  }
+
+  PlaneLoad<VX, TY> next;  // plane z + 1 (and p of plane z), loaded one iteration ahead
+  load_plane<VX, TY>(a, z0 + 1, y0, xb, vec, next);
+  for (int z = z0; z < z1; ++z) {
+    const PlaneLoad<VX, TY> cur = next;
+    if (z + 1 < z1) load_plane<VX, TY>(a, z + 2, y0, xb, vec, next);
+    primal_plane<VX, TY>(a, z + 1, y0, xb, cur.r, cur.ry_top, r[0], cur.b, zn);
 
 #pragma unroll
     for (int j = 0; j < TY; ++j) {
@@ -189,7 +209,7 @@ __global__ void __launch_bounds__(64) k_fgp_step(TvArgs a, int chunks) {
         g[0] = z < a.nz - 1 ? c - zn[j][v] : 0.0f;
         g[1] = y < a.ny - 1 ? c - zc[j + 1][v] : 0.0f;
         g[2] = x < a.nx - 1 ? c - (v == VX - 1 ? right : zc[j][v + 1]) : 0.0f;
-        float p3[3] = {pv[0][j][v], pv[1][j][v], pv[2][j][v]};
+        float p3[3] = {cur.p[0][j][v], cur.p[1][j][v], cur.p[2][j][v]};
         float r3[3] = {r[0][j][v], r[1][j][v], r[2][j][v]};
         tv_dual_step(g, a.step, a.momentum, p3, r3);
 #pragma unroll
@@ -210,7 +230,7 @@ __global__ void __launch_bounds__(64) k_fgp_step(TvArgs a, int chunks) {
       for (int v = 0; v < VX; ++v) {
         zc[j][v] = zn[j][v];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) r[c][j][v] = rn[c][j][v];
+        for (int c = 0; c < 3; ++c) r[c][j][v] = cur.r[c][j][v];
       }
     }
   }
