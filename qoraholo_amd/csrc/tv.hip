@@ -4,9 +4,14 @@
 // One wavefront per block owns a strip of TY rows of the volume, each lane VX consecutive x
 // positions of every row (a whole 256-wide row is 64 lanes x 4 floats, loaded as float4), and
 // marches along z through a chunk of planes. y neighbours are in the lane's own registers, x
-// neighbours one lane away (shuffles), z neighbours in the registers of the previous plane, so
-// every voxel's b, p and r are read once per step and p and r written once; only one halo row
-// above and below the strip (and one plane at the chunk start) is read twice.
+// neighbours one lane away (shuffles), z neighbours in the registers of the previous plane; only
+// one halo row above and below the strip (and one plane at the chunk start) is read twice.
+//
+// The extrapolated dual r_k = p_k + m_{k-1} (p_k - p_{k-1}) is not stored: each step reads the
+// two latest dual iterates and forms r with the same float operations the reference's dual step
+// uses to produce it (src/common.hpp:128-141), so a step reads b, p_k, p_{k-1} and writes p_{k+1}
+// (10 volumes instead of 13 with r stored). The first step of a prox uses r = p_k
+// (src/cpu/tv.cpp:10).
 #include "common.hpp"
 #include "runtime.hpp"
 #include "tv.hpp"
@@ -18,16 +23,16 @@ namespace {
 constexpr int kZChunk = 16;
 
 struct TvArgs {
-  const float* b;     // primal data, dense (nz, ny, nx)
-  const float* r_in;  // dual extrapolation r, three components `vol` floats apart
-  const float* p_in;  // dual p at the start of the step
-  float* p_out;       // dual p after the step (p_in or another buffer)
-  float* r_out;       // r after the step (never r_in)
+  const float* b;       // primal data, dense (nz, ny, nx)
+  const float* p_in;    // dual p_k, three components `vol` floats apart
+  const float* p_prev;  // dual p_{k-1} (unused on the first step)
+  float* p_out;         // dual p_{k+1}, a third buffer
   long vol;
   int nz, ny, nx;
   float lambda;
   float step;
-  float momentum;
+  float momentum;  // m_{k-1}, the momentum of the step that produced p_k
+  int first;       // r = p_k exactly
 };
 
 // Block b runs tile t: with a grid that is a multiple of 8, blocks b, b+8, b+16, ... (which the
@@ -131,32 +136,50 @@ __device__ __forceinline__ void primal_plane(const TvArgs& a, int zp, int y0, in
   }
 }
 
-// Data of one plane that the march loads ahead: r on the strip rows and the halo row below,
-// the y component of r on the row above, b on the same rows, and p on the strip rows of the
-// plane before it (the plane whose dual step runs while this plane's primal is formed).
-template <int VX, int TY>
-struct PlaneLoad {
-  float r[3][TY + 1][VX];
-  float ry_top[VX];
-  float b[TY + 1][VX];
-  float p[3][TY][VX];
-};
+// r = p_k + m (p_k - p_{k-1}) on one row (or p_k on the first step of a prox).
+template <int VX>
+__device__ __forceinline__ void extrapolate(const TvArgs& a, const float (&pk)[VX],
+                                            const float (&pm)[VX], float (&r)[VX]) {
+#pragma unroll
+  for (int v = 0; v < VX; ++v) r[v] = a.first ? pk[v] : pk[v] + a.momentum * (pk[v] - pm[v]);
+}
 
+// r of plane z on the strip rows and the halo row below (R), its y component on the row above
+// (ry_top), and b on the strip rows and the halo row. P keeps p_k of the strip rows for the
+// dual step of this plane.
 template <int VX, int TY>
 __device__ __forceinline__ void load_plane(const TvArgs& a, int z, int y0, int xb, bool vec,
-                                           PlaneLoad<VX, TY>& L) {
+                                           float (&R)[3][TY + 1][VX], float (&ry_top)[VX],
+                                           float (&B)[TY + 1][VX], float (&P)[3][TY][VX]) {
 #pragma unroll
   for (int j = 0; j <= TY; ++j) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) load_row<VX>(a.r_in + c * a.vol, a, z, y0 + j, xb, vec, L.r[c][j]);
-    load_row<VX>(a.b, a, z, y0 + j, xb, vec, L.b[j]);
-  }
-  load_row<VX>(a.r_in + a.vol, a, z, y0 - 1, xb, vec, L.ry_top);
+    for (int c = 0; c < 3; ++c) {
+      float pk[VX], pm[VX];
+      load_row<VX>(a.p_in + c * a.vol, a, z, y0 + j, xb, vec, pk);
+      if (!a.first) {
+        load_row<VX>(a.p_prev + c * a.vol, a, z, y0 + j, xb, vec, pm);
+      } else {
 #pragma unroll
-  for (int j = 0; j < TY; ++j) {
+        for (int v = 0; v < VX; ++v) pm[v] = 0.0f;
+      }
+      extrapolate<VX>(a, pk, pm, R[c][j]);
+      if (j < TY) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) load_row<VX>(a.p_in + c * a.vol, a, z - 1, y0 + j, xb, vec, L.p[c][j]);
+        for (int v = 0; v < VX; ++v) P[c][j][v] = pk[v];
+      }
+    }
+    load_row<VX>(a.b, a, z, y0 + j, xb, vec, B[j]);
   }
+  float pk[VX], pm[VX];
+  load_row<VX>(a.p_in + a.vol, a, z, y0 - 1, xb, vec, pk);
+  if (!a.first) {
+    load_row<VX>(a.p_prev + a.vol, a, z, y0 - 1, xb, vec, pm);
+  } else {
+#pragma unroll
+    for (int v = 0; v < VX; ++v) pm[v] = 0.0f;
+  }
+  extrapolate<VX>(a, pk, pm, ry_top);
 }
 
 template <int VX, int TY>
@@ -169,38 +192,36 @@ __global__ void __launch_bounds__(64) k_fgp_step(TvArgs a, int chunks) {
   const bool vec = (a.nx % 4) == 0;
 
   float r[3][TY + 1][VX];  // r at plane z, rows y0..y0+TY
+  float pc[3][TY][VX];     // p_k at plane z, strip rows
   float zc[TY + 1][VX];    // primal at plane z
-  float zn[TY + 1][VX];
-
-  // Warm-up: the primal on plane z0 needs r_z on plane z0 - 1.
   {
+    // Warm-up: the primal on plane z0 needs the z component of r on plane z0 - 1.
     float rz_below[TY + 1][VX];
-    float bb[TY + 1][VX];
-    float ry_top[VX];
 #pragma unroll
     for (int j = 0; j <= TY; ++j) {
-      load_row<VX>(a.r_in, a, z0 - 1, y0 + j, xb, vec, rz_below[j]);
+      float pk[VX], pm[VX];
+      load_row<VX>(a.p_in, a, z0 - 1, y0 + j, xb, vec, pk);
+      if (!a.first) {
+        load_row<VX>(a.p_prev, a, z0 - 1, y0 + j, xb, vec, pm);
+      } else {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) load_row<VX>(a.r_in + c * a.vol, a, z0, y0 + j, xb, vec, r[c][j]);
-      load_row<VX>(a.b, a, z0, y0 + j, xb, vec, bb[j]);
+        for (int v = 0; v < VX; ++v) pm[v] = 0.0f;
+      }
+      extrapolate<VX>(a, pk, pm, rz_below[j]);
     }
-    load_row<VX>(a.r_in + a.vol, a, z0, y0 - 1, xb, vec, ry_top);
+    float ry_top[VX], bb[TY + 1][VX];
+    load_plane<VX, TY>(a, z0, y0, xb, vec, r, ry_top, bb, pc);
     primal_plane<VX, TY>(a, z0, y0, xb, r, ry_top, rz_below, bb, zc);
   }
-
-  PlaneLoad<VX, TY> next;  // plane z + 1 (and p of plane z), loaded one iteration ahead
-  load_plane<VX, TY>(a, z0 + 1, y0, xb, vec, next);
   for (int z = z0; z < z1; ++z) {
-    const PlaneLoad<VX, TY> cur = next;
-    if (z + 1 < z1) load_plane<VX, TY>(a, z + 2, y0, xb, vec, next);
-    primal_plane<VX, TY>(a, z + 1, y0, xb, cur.r, cur.ry_top, r[0], cur.b, zn);
-
+    float rn[3][TY + 1][VX], ryn[VX], bn[TY + 1][VX], pn[3][TY][VX], zn[TY + 1][VX];
+    load_plane<VX, TY>(a, z + 1, y0, xb, vec, rn, ryn, bn, pn);
+    primal_plane<VX, TY>(a, z + 1, y0, xb, rn, ryn, r[0], bn, zn);
 #pragma unroll
     for (int j = 0; j < TY; ++j) {
       const int y = y0 + j;
       const float right = __shfl_down(zc[j][0], 1);
       float po[3][VX];
-      float ro[3][VX];
 #pragma unroll
       for (int v = 0; v < VX; ++v) {
         const int x = xb + v;
@@ -209,20 +230,14 @@ __global__ void __launch_bounds__(64) k_fgp_step(TvArgs a, int chunks) {
         g[0] = z < a.nz - 1 ? c - zn[j][v] : 0.0f;
         g[1] = y < a.ny - 1 ? c - zc[j + 1][v] : 0.0f;
         g[2] = x < a.nx - 1 ? c - (v == VX - 1 ? right : zc[j][v + 1]) : 0.0f;
-        float p3[3] = {cur.p[0][j][v], cur.p[1][j][v], cur.p[2][j][v]};
+        float p3[3] = {pc[0][j][v], pc[1][j][v], pc[2][j][v]};
         float r3[3] = {r[0][j][v], r[1][j][v], r[2][j][v]};
-        tv_dual_step(g, a.step, a.momentum, p3, r3);
+        tv_dual_step(g, a.step, 0.0f, p3, r3);
 #pragma unroll
-        for (int cc = 0; cc < 3; ++cc) {
-          po[cc][v] = p3[cc];
-          ro[cc][v] = r3[cc];
-        }
+        for (int cc = 0; cc < 3; ++cc) po[cc][v] = p3[cc];
       }
 #pragma unroll
-      for (int cc = 0; cc < 3; ++cc) {
-        store_row<VX>(a.p_out + cc * a.vol, a, z, y, xb, vec, po[cc]);
-        store_row<VX>(a.r_out + cc * a.vol, a, z, y, xb, vec, ro[cc]);
-      }
+      for (int cc = 0; cc < 3; ++cc) store_row<VX>(a.p_out + cc * a.vol, a, z, y, xb, vec, po[cc]);
     }
 #pragma unroll
     for (int j = 0; j <= TY; ++j) {
@@ -230,7 +245,15 @@ __global__ void __launch_bounds__(64) k_fgp_step(TvArgs a, int chunks) {
       for (int v = 0; v < VX; ++v) {
         zc[j][v] = zn[j][v];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) r[c][j][v] = cur.r[c][j][v];
+        for (int c = 0; c < 3; ++c) r[c][j][v] = rn[c][j][v];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < TY; ++j) {
+#pragma unroll
+      for (int v = 0; v < VX; ++v) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) pc[c][j][v] = pn[c][j][v];
       }
     }
   }
@@ -316,10 +339,11 @@ dim3 strip_grid(int ny, int nz, int* chunks) {
 
 }  // namespace
 
-void fgp_step(const float* b, const float* r_in, const float* p_in, float* p_out, float* r_out,
-              int nz, int ny, int nx, float lambda, float step, float momentum, hipStream_t s) {
-  TvArgs a{b, r_in, p_in, p_out, r_out, static_cast<long>(nz) * ny * nx, nz, ny, nx,
-           lambda, step, momentum};
+void fgp_step(const float* b, const float* p_in, const float* p_prev, float* p_out, int nz,
+              int ny, int nx, float lambda, float step, float momentum, bool first,
+              hipStream_t s) {
+  TvArgs a{b, p_in, p_prev, p_out, static_cast<long>(nz) * ny * nx, nz, ny, nx,
+           lambda, step, momentum, first ? 1 : 0};
   int chunks = 0;
   const dim3 grid = strip_grid(ny, nz, &chunks);
   switch (lanes_x(nx)) {
@@ -334,8 +358,8 @@ void fgp_step(const float* b, const float* r_in, const float* p_in, float* p_out
 
 void tv_primal(const float* b, const float* p, int nz, int ny, int nx, float lambda, float* out,
                bool to_index, float nm2, float inv_k02, hipStream_t s) {
-  TvArgs a{b, nullptr, p, nullptr, nullptr, static_cast<long>(nz) * ny * nx, nz, ny, nx,
-           lambda, 0.0f, 0.0f};
+  TvArgs a{b, p, nullptr, nullptr, static_cast<long>(nz) * ny * nx, nz, ny, nx,
+           lambda, 0.0f, 0.0f, 1};
   int chunks = 0;
   const dim3 grid = strip_grid(ny, nz, &chunks);
   const int ti = to_index ? 1 : 0;
@@ -351,41 +375,39 @@ void tv_primal(const float* b, const float* p, int nz, int ny, int nx, float lam
 
 TvState::TvState(int nz, int ny, int nx)
     : nz_(nz), ny_(ny), nx_(nx), vol_(static_cast<long>(nz) * ny * nx),
-      p_{DeviceBuffer<float>(3 * vol_), DeviceBuffer<float>(3 * vol_)},
-      r_{DeviceBuffer<float>(3 * vol_), DeviceBuffer<float>(3 * vol_)} {}
+      p_{DeviceBuffer<float>(3 * vol_), DeviceBuffer<float>(3 * vol_),
+         DeviceBuffer<float>(3 * vol_)} {}
 
 void TvState::reset(hipStream_t s) {
   fill(p_[0].get(), 3 * vol_, 0.0f, s);
   cur_ = 0;
 }
 
-std::size_t TvState::bytes() const {
-  return p_[0].bytes() + p_[1].bytes() + r_[0].bytes() + r_[1].bytes();
-}
+std::size_t TvState::bytes() const { return p_[0].bytes() + p_[1].bytes() + p_[2].bytes(); }
 
 // tv_prox of the reference (src/cpu/tv.cpp:7-44): r <- p, `inner` FGP steps, out = primal(p).
-// The copy r <- p is folded into the first step, which reads p as r and writes the new p to
-// the second p buffer; later steps update p in place and ping-pong r.
+// The dual iterates rotate through three buffers: step k reads p_k and p_{k-1} and writes p_{k+1}
+// to the third; r is formed inside the step. The prox leaves p in buffer cur_ for the next call.
 void TvState::prox(const float* b, float* out, float lambda, int inner, hipStream_t s,
                    bool to_index, float nm2, float inv_k02, Event* t0, Event* t1) {
   const float step = 1.0f / (12.0f * lambda);
   float t = 1.0f;
-  float* p_old = p_[cur_].get();
-  float* p_new = p_[1 - cur_].get();
-  const float* r_in = p_old;
-  int rs = 0;
+  float m_prev = 0.0f;
+  int cur = cur_;
+  int prev = (cur_ + 2) % 3;
   if (t0) t0->record(s);
   for (int k = 0; k < inner; ++k) {
+    const int next = 3 - cur - prev;
+    fgp_step(b, p_[cur].get(), p_[prev].get(), p_[next].get(), nz_, ny_, nx_, lambda, step,
+             m_prev, k == 0, s);
     const float tn = fista_t(t);
-    const float m = (t - 1.0f) / tn;
-    float* r_out = r_[rs].get();
-    fgp_step(b, r_in, k == 0 ? p_old : p_new, p_new, r_out, nz_, ny_, nx_, lambda, step, m, s);
-    r_in = r_out;
-    rs = 1 - rs;
+    m_prev = (t - 1.0f) / tn;
     t = tn;
+    prev = cur;
+    cur = next;
   }
   if (t1) t1->record(s);
-  if (inner > 0) cur_ = 1 - cur_;
+  cur_ = cur;
   tv_primal(b, p_[cur_].get(), nz_, ny_, nx_, lambda, out, to_index, nm2, inv_k02, s);
 }
 

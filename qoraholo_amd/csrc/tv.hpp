@@ -12,18 +12,19 @@ namespace qh {
 // d[0, n) = v, by a kernel (safe inside captured graphs, unlike a memset node).
 void fill(float* d, long n, float v, hipStream_t s);
 
-// One fused FGP dual step over a dense (nz, ny, nx) volume: z = max(b - lambda L(r_in), 0)
-// at each voxel and its upper neighbours, g = L^T z, then the projected dual update of p and
-// the extrapolation r_out. p_in may equal p_out; r_in never equals r_out.
-void fgp_step(const float* b, const float* r_in, const float* p_in, float* p_out, float* r_out,
-              int nz, int ny, int nx, float lambda, float step, float momentum, hipStream_t s);
+// One fused FGP dual step over a dense (nz, ny, nx) volume: r = p_k + m (p_k - p_prev) (r = p_k
+// on the first step), z = max(b - lambda L(r), 0) at each voxel and its upper neighbours,
+// g = L^T z, and the projected dual update p_out = P(r + step g). p_out is a third buffer.
+void fgp_step(const float* b, const float* p_in, const float* p_prev, float* p_out, int nz,
+              int ny, int nx, float lambda, float step, float momentum, bool first,
+              hipStream_t s);
 
 // out = max(b - lambda L(p), 0); with to_index the refractive index of that potential.
 void tv_primal(const float* b, const float* p, int nz, int ny, int nx, float lambda, float* out,
                bool to_index, float nm2, float inv_k02, hipStream_t s);
 
-// Dual state of the TV prox, kept across calls (FISTA warm start): two p buffers and two r
-// buffers of three components each.
+// Dual state of the TV prox, kept across calls (FISTA warm start): three rotating dual buffers
+// of three components each.
 class TvState {
  public:
   TvState(int nz, int ny, int nx);
@@ -38,8 +39,7 @@ class TvState {
  private:
   int nz_, ny_, nx_;
   long vol_;
-  DeviceBuffer<float> p_[2];
-  DeviceBuffer<float> r_[2];
+  DeviceBuffer<float> p_[3];
   int cur_ = 0;
 };
 
