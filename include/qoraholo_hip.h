@@ -189,8 +189,9 @@ const float* qh_hip_pipeline_output(const qh_pipeline* p);
 /* Reconstructs from input() into output(); QH_ERR_LOGIC without a background. */
 int qh_hip_pipeline_process(qh_pipeline* p);
 /* The same from holograms already in device memory into a device volume, on `stream`
- * (a hipStream_t; NULL for the pipeline's own). Returns once the work is queued; the caller
- * synchronises the stream before reading ri_device. */
+ * (a hipStream_t, used as given: NULL is the null stream, which is also torch's default
+ * stream). Returns once the work is queued; the caller synchronises the stream before reading
+ * ri_device. */
 int qh_hip_pipeline_process_device(qh_pipeline* p, const float* holograms_device,
                                    float* ri_device, void* stream);
 double qh_hip_pipeline_filled(const qh_pipeline* p);
@@ -239,6 +240,70 @@ int qh_hip_server_wait(qh_server* s, int64_t id, double* filled, double* latency
 int qh_hip_server_poll(qh_server* s, int64_t id, int* done);
 int qh_hip_server_stats(qh_server* s, qh_server_stats* out);
 int qh_hip_server_reset_stats(qh_server* s);
+
+/* ---- beam propagation (replaces cuda::bpm_forward / bpm_reconstruct / bpm_vjp,
+ *      reference src/backends.hpp:119-124, and the tables of src/bpm.cpp:10-60) ----
+ * Volumes are the contrast x = ri - n_medium (nz, n, n), as the reference's backends receive
+ * it (src/api.cpp:159-163); fields and residuals are (count, n, n). Lateral boundaries are
+ * periodic; n must be even and of the form 2^a 3^b 5^c, at most 2048. */
+
+/* Reference BpmOptions (include/qoraholo/bpm.hpp:18-25) plus the device layout. */
+typedef struct {
+  int obliquity;  /* weight the scattered part of each slice by its obliquity (reference: 1) */
+  int chunk;      /* illuminations per device pass; 0: as many as keep the adjoint's buffers
+                   * within half the free device memory */
+  int checkpoint; /* slices per recomputed segment of the adjoint; 0: keep every envelope */
+} qh_bpm_options;
+
+/* Reference BpmSolveOptions (include/qoraholo/bpm.hpp:32-45). */
+typedef struct {
+  int iterations;
+  int batch; /* illuminations per gradient step; 0 or count: all, with FISTA momentum */
+  float step; /* multiple of 1 / L */
+  float tv_weight; /* 0: n >= n_medium only; > 0: TV prox on the contrast (n <= 512) */
+  int tv_inner;
+} qh_bpm_solve_options;
+
+/* The tables in float as bpm_tables (src/bpm.cpp:10-60) stores them: step, last (count, n, n)
+ * complex, q (count, n, n), the screen phase k0 pixel and the Lipschitz constant. Host only. */
+int qh_bpm_tables(int n, int nz, const qh_illumination* illum, int count,
+                  const qh_geometry* geometry, int obliquity, qh_c64* step, qh_c64* last,
+                  float* q, float* phase, float* lipschitz);
+/* cuda::bpm_forward (src/backends.hpp:119): x (nz, n, n) -> field ratios (count, n, n) at z = 0. */
+int qh_hip_bpm_forward(const float* x, int nz, int n, const qh_illumination* illum, int count,
+                       const qh_geometry* geometry, const qh_bpm_options* options, qh_c64* out);
+/* cuda::bpm_vjp (src/backends.hpp:124): grad (nz, n, n) = Re(J^H r) at x. */
+int qh_hip_bpm_vjp(const float* x, const qh_c64* r, int nz, int n, const qh_illumination* illum,
+                   int count, const qh_geometry* geometry, const qh_bpm_options* options,
+                   float* grad);
+/* cuda::bpm_reconstruct (src/backends.hpp:122-123): x holds the initial contrast (nz, n, n) and
+ * receives the result's; loss receives solve->iterations data-term values. */
+int qh_hip_bpm_reconstruct(const qh_c64* fields, int count, int n, const qh_illumination* illum,
+                           const qh_geometry* geometry, float* x, int nz,
+                           const qh_bpm_options* options, const qh_bpm_solve_options* solve,
+                           double* loss);
+
+/* Device-resident model for callers that keep x, fields and gradients in device memory (e.g.
+ * a torch.autograd.Function): tables and work buffers stay on the device between calls. The
+ * _dev calls take device pointers and a stream (a hipStream_t, used as given: NULL is the null
+ * stream, torch's default), queue their work and return; the caller synchronises the stream.
+ * A plan is used by one thread at a time. */
+typedef struct qh_bpm_plan qh_bpm_plan;
+int qh_hip_bpm_plan_create(int nz, int n, const qh_illumination* illum, int count,
+                           const qh_geometry* geometry, const qh_bpm_options* options,
+                           qh_bpm_plan** out);
+void qh_hip_bpm_plan_destroy(qh_bpm_plan* plan);
+/* Screen phase, Lipschitz constant, illuminations per pass and device bytes of the plan. */
+int qh_hip_bpm_plan_info(const qh_bpm_plan* plan, float* phase, float* lipschitz, int* chunk,
+                         size_t* device_bytes);
+int qh_hip_bpm_forward_dev(qh_bpm_plan* plan, const float* x, qh_c64* out, void* stream);
+int qh_hip_bpm_vjp_dev(qh_bpm_plan* plan, const float* x, const qh_c64* r, float* grad,
+                       void* stream);
+/* gradient() of src/cpu/bpm.cpp:100-129: grad = Re(J^H (bpm(x) - fields)) / (last - first)
+ * over illuminations [first, last); *loss = 0.5 / count times the sum of squared residuals.
+ * Synchronises the stream to return the loss. */
+int qh_hip_bpm_gradient_dev(qh_bpm_plan* plan, const float* x, const qh_c64* fields, int first,
+                            int last, float* grad, double* loss, void* stream);
 
 /* ---- transform and TV test hooks (no counterpart in the reference's ABI: it used cuFFT) ---- */
 

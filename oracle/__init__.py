@@ -214,3 +214,103 @@ def pipeline(holograms, backgrounds, cy, cx, radius, out_size, illuminations, ge
     ratio = field_ratio(extract_fields(holograms, cy, cx, radius, out_size, dtype),
                         extract_fields(backgrounds, cy, cx, radius, out_size, dtype), dtype)
     return reconstruct(rytov(ratio, dtype), illuminations, geometry, dtype=dtype, **odt)
+
+
+# ---- beam propagation model (oracle/bpm.cpp) ----
+
+def _bpm_check(rc: int) -> None:
+    if rc != 0:
+        lib().oqh_bpm_last_error.restype = ctypes.c_char_p
+        raise RuntimeError(lib().oqh_bpm_last_error().decode())
+
+
+def _bpm_common(illuminations, geometry):
+    il = np.ascontiguousarray(np.reshape(illuminations, (-1, 2)), dtype=np.float32)
+    return il, np.asarray(geometry, dtype=np.float32)
+
+
+def bpm_tables(n, nz, illuminations, geometry, obliquity=True):
+    """(step, last, q, phase, lipschitz) as the reference stores them (float)."""
+    il, geom = _bpm_common(illuminations, geometry)
+    count = il.shape[0]
+    step = np.empty((count, n, n), np.complex64)
+    last = np.empty((count, n, n), np.complex64)
+    q = np.empty((count, n, n), np.float32)
+    sc = np.empty(2, np.float32)
+    _bpm_check(lib().oqh_bpm_tables_f(n, nz, _ptr(il), count, _ptr(geom), int(obliquity),
+                                      _ptr(step), _ptr(last), _ptr(q), _ptr(sc)))
+    return step, last, q, float(sc[0]), float(sc[1])
+
+
+def bpm_forward(x, illuminations, geometry, obliquity=True, dtype=np.float32):
+    """Contrast x = ri - n_medium (nz, n, n) -> field ratios (count, n, n) at z = 0."""
+    x = np.ascontiguousarray(x, dtype=dtype)
+    il, geom = _bpm_common(illuminations, geometry)
+    nz, n, _ = x.shape
+    out = np.empty((il.shape[0], n, n), _cdt(dtype))
+    _bpm_check(getattr(lib(), f"oqh_bpm_forward_{_suffix(dtype)}")(
+        _ptr(x), nz, n, _ptr(il), il.shape[0], _ptr(geom), int(obliquity), _ptr(out)))
+    return out
+
+
+def bpm_jvp(x, h, illuminations, geometry, obliquity=True, dtype=np.float32):
+    """Tangent-linear model J h at x."""
+    x = np.ascontiguousarray(x, dtype=dtype)
+    h = np.ascontiguousarray(h, dtype=dtype)
+    il, geom = _bpm_common(illuminations, geometry)
+    nz, n, _ = x.shape
+    out = np.empty((il.shape[0], n, n), _cdt(dtype))
+    _bpm_check(getattr(lib(), f"oqh_bpm_jvp_{_suffix(dtype)}")(
+        _ptr(x), _ptr(h), nz, n, _ptr(il), il.shape[0], _ptr(geom), int(obliquity), _ptr(out)))
+    return out
+
+
+def bpm_vjp(x, r, illuminations, geometry, obliquity=True, dtype=np.float32):
+    """Re(J^H r) at x, summed over illuminations in order."""
+    x = np.ascontiguousarray(x, dtype=dtype)
+    r = np.ascontiguousarray(r, dtype=_cdt(dtype))
+    il, geom = _bpm_common(illuminations, geometry)
+    nz, n, _ = x.shape
+    grad = np.empty_like(x)
+    _bpm_check(getattr(lib(), f"oqh_bpm_vjp_{_suffix(dtype)}")(
+        _ptr(x), _ptr(r), nz, n, _ptr(il), il.shape[0], _ptr(geom), int(obliquity), _ptr(grad)))
+    return grad
+
+
+def bpm_gradient(x, fields, illuminations, geometry, first=0, last=None, obliquity=True,
+                 dtype=np.float32):
+    """(gradient of 0.5/(last-first) sum |bpm(x) - fields|^2, data term with 0.5/count)."""
+    x = np.ascontiguousarray(x, dtype=dtype)
+    f = np.ascontiguousarray(fields, dtype=_cdt(dtype))
+    il, geom = _bpm_common(illuminations, geometry)
+    nz, n, _ = x.shape
+    count = il.shape[0]
+    last = count if last is None else last
+    grad = np.empty_like(x)
+    loss = ctypes.c_double()
+    _bpm_check(getattr(lib(), f"oqh_bpm_gradient_{_suffix(dtype)}")(
+        _ptr(x), _ptr(f), nz, n, _ptr(il), count, _ptr(geom), int(obliquity), first, last,
+        _ptr(grad), ctypes.byref(loss)))
+    return grad, loss.value
+
+
+def bpm_reconstruct(fields, illuminations, geometry, initial, iterations=20, batch=0, step=1.0,
+                    tv_weight=0.0, tv_inner=10, obliquity=True, dtype=np.float32):
+    """Field ratios -> index volume by the reference's BPM inversion; returns (ri, loss).
+
+    ``initial`` is a refractive-index volume; the contrast is formed in float32 as the
+    reference's api.cpp:159-163 does, and n_medium is added back at the end (api.cpp:199)."""
+    f = np.ascontiguousarray(fields, dtype=_cdt(dtype))
+    il, geom = _bpm_common(illuminations, geometry)
+    count, n, _ = f.shape
+    nm = np.float32(geom[1])
+    x = (np.asarray(initial, np.float32) - nm).astype(dtype)
+    x = np.ascontiguousarray(x)
+    nz = x.shape[0]
+    opts = np.array([iterations, batch, tv_inner, int(obliquity)], np.int32)
+    loss = np.zeros(iterations, np.float64)
+    _bpm_check(getattr(lib(), f"oqh_bpm_reconstruct_{_suffix(dtype)}")(
+        _ptr(f), count, n, _ptr(il), _ptr(geom), _ptr(x), nz, _ptr(opts), ctypes.c_float(step),
+        ctypes.c_float(tv_weight), _ptr(loss)))
+    ri = (x.astype(np.float32) + nm) if np.dtype(dtype) == np.float32 else x + float(nm)
+    return ri, loss

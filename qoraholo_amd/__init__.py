@@ -16,15 +16,18 @@ hold two runtimes.
 from __future__ import annotations
 
 import ctypes
+import sys
 from dataclasses import dataclass
 
 import numpy as np
 
 from . import _native
 from ._native import LogicError, OutOfDeviceMemory
+from .bpm import BpmPlan, bpm, bpm_forward, bpm_reconstruct, bpm_tables, bpm_vjp
 from .phantom import Ellipsoid, Sphere, forward_fields, render_phantom, synthesize_holograms
 
 __all__ = [
+    "BpmPlan",
     "Ellipsoid",
     "Geometry",
     "LogicError",
@@ -33,6 +36,11 @@ __all__ = [
     "Server",
     "Sideband",
     "Sphere",
+    "bpm",
+    "bpm_forward",
+    "bpm_reconstruct",
+    "bpm_tables",
+    "bpm_vjp",
     "circular_scan",
     "cuda_available",
     "extract_fields",
@@ -388,7 +396,8 @@ class Pipeline:
 
     def run_device(self, holograms, out, stream=None) -> None:
         """Holograms in device memory -> index volume in device memory, queued on `stream`
-        (a torch stream, a hipStream_t address, or None for the pipeline's own stream)."""
+        (a torch stream or a hipStream_t address; None: torch's current stream when torch is
+        loaded, else the null stream)."""
         src = holograms.data_ptr() if hasattr(holograms, "data_ptr") else int(holograms)
         dst = out.data_ptr() if hasattr(out, "data_ptr") else int(out)
         if hasattr(holograms, "numel"):
@@ -396,10 +405,12 @@ class Pipeline:
                 raise ValueError("Pipeline::run_device: holograms must be (count, N, N)")
             if out.numel() != int(np.prod(self.output_shape)):
                 raise ValueError("Pipeline::run_device: out must be (nz, n, n)")
-        s = stream.cuda_stream if hasattr(stream, "cuda_stream") else stream
+        if stream is None:
+            torch = sys.modules.get("torch")
+            stream = torch.cuda.current_stream() if torch is not None else 0
+        s = stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream)
         _native.check(_native.lib().qh_hip_pipeline_process_device(
-            self._handle, ctypes.c_void_p(src), ctypes.c_void_p(dst),
-            ctypes.c_void_p(s) if s else None))
+            self._handle, ctypes.c_void_p(src), ctypes.c_void_p(dst), ctypes.c_void_p(s)))
 
     def time_fgp(self, on: bool = True) -> None:
         """Record the FGP kernel intervals in ``timings`` (fgp_ms, fgp_steps)."""

@@ -65,6 +65,15 @@ class ServerStats_t(ctypes.Structure):
                 ("busy_slots", c_int)]
 
 
+class BpmOptions_t(ctypes.Structure):
+    _fields_ = [("obliquity", c_int), ("chunk", c_int), ("checkpoint", c_int)]
+
+
+class BpmSolveOptions_t(ctypes.Structure):
+    _fields_ = [("iterations", c_int), ("batch", c_int), ("step", c_float),
+                ("tv_weight", c_float), ("tv_inner", c_int)]
+
+
 class PipelineOptions_t(ctypes.Structure):
     _fields_ = [("count", c_int), ("hologram_size", c_int), ("crop", CropPlan_t),
                 ("illuminations", POINTER(Illumination_t)), ("geometry", Geometry_t),
@@ -108,6 +117,24 @@ _SIGNATURES = {
     "qh_hip_server_poll": (c_int, [c_void_p, ctypes.c_int64, POINTER(c_int)]),
     "qh_hip_server_stats": (c_int, [c_void_p, POINTER(ServerStats_t)]),
     "qh_hip_server_reset_stats": (c_int, [c_void_p]),
+    "qh_bpm_tables": (c_int, [c_int, c_int, c_void_p, c_int, POINTER(Geometry_t), c_int, c_void_p,
+                              c_void_p, c_void_p, POINTER(c_float), POINTER(c_float)]),
+    "qh_hip_bpm_forward": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, POINTER(Geometry_t),
+                                   POINTER(BpmOptions_t), c_void_p]),
+    "qh_hip_bpm_vjp": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_int,
+                               POINTER(Geometry_t), POINTER(BpmOptions_t), c_void_p]),
+    "qh_hip_bpm_reconstruct": (c_int, [c_void_p, c_int, c_int, c_void_p, POINTER(Geometry_t),
+                                       c_void_p, c_int, POINTER(BpmOptions_t),
+                                       POINTER(BpmSolveOptions_t), c_void_p]),
+    "qh_hip_bpm_plan_create": (c_int, [c_int, c_int, c_void_p, c_int, POINTER(Geometry_t),
+                                       POINTER(BpmOptions_t), POINTER(c_void_p)]),
+    "qh_hip_bpm_plan_destroy": (None, [c_void_p]),
+    "qh_hip_bpm_plan_info": (c_int, [c_void_p, POINTER(c_float), POINTER(c_float), POINTER(c_int),
+                                     POINTER(c_size_t)]),
+    "qh_hip_bpm_forward_dev": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "qh_hip_bpm_vjp_dev": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "qh_hip_bpm_gradient_dev": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p,
+                                        POINTER(c_double), c_void_p]),
     "qh_hip_fft_c2c": (c_int, [c_void_p, c_int, c_int, c_int, c_uint, c_int, c_float]),
     "qh_hip_rfft3": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p]),
     "qh_hip_irfft3": (c_int, [c_void_p, c_int, c_int, c_int, c_float, c_void_p]),

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/qoraholo_hip.h"
+#include "bpm.hpp"
 #include "fft.hpp"
 #include "pipeline.hpp"
 #include "runtime.hpp"
@@ -23,6 +24,10 @@ struct qh_pipeline {
 
 struct qh_server {
   std::unique_ptr<qh::Server> impl;
+};
+
+struct qh_bpm_plan {
+  std::unique_ptr<qh::BpmModel> impl;
 };
 
 namespace qh {
@@ -114,6 +119,25 @@ void check_plan(const qh_crop_plan& p, int n) {
           "extract_fields: output_size must be even and <= N");
   require(p.radius > 0 && p.radius <= static_cast<float>(p.out) / 2.0f,
           "extract_fields: crop radius must fit in the output window");
+}
+
+// src/api.cpp:141-157 (checked_bpm_tables): checks of bpm_forward and bpm_reconstruct.
+BpmTables checked_bpm_tables(int nz, int n, int count, const qh_illumination* illum,
+                             const qh_geometry* g, const qh_bpm_options* o, const char* who) {
+  const std::string w(who);
+  require(g != nullptr && o != nullptr, w + ": missing geometry or options");
+  require(n >= 4 && n % 2 == 0 && nz >= 1, w + ": volume must be (nz, n, n) with an even n");
+  require(count >= 1 && illum != nullptr, w + ": need one illumination per field");
+  require(g->wavelength > 0 && g->pixel > 0 && g->n_medium > 0 && g->na > 0,
+          w + ": geometry values must be positive");
+  const float fm = g->n_medium / g->wavelength;
+  for (int a = 0; a < count; ++a) {
+    require(illum[a].fy * illum[a].fy + illum[a].fx * illum[a].fx < fm * fm,
+            w + ": illumination frequency beyond n_medium / wavelength");
+  }
+  require(o->chunk >= 0 && o->checkpoint >= 0, w + ": chunk and checkpoint must not be negative");
+  require_fft(n, who);
+  return bpm_tables(n, nz, illum, count, *g, o->obliquity != 0);
 }
 
 }  // namespace
@@ -560,6 +584,163 @@ int qh_hip_irfft3(const qh_c64* in, int nz, int ny, int nx, float scale, float* 
     irfft3(dh.get(), dx.get(), nz, ny, nx, scale, s);
     check(hipMemcpyAsync(x, dx.get(), dx.bytes(), hipMemcpyDeviceToHost, s), "download");
     check(hipStreamSynchronize(s), "irfft3");
+  });
+}
+
+int qh_bpm_tables(int n, int nz, const qh_illumination* illum, int count,
+                  const qh_geometry* geometry, int obliquity, qh_c64* step, qh_c64* last,
+                  float* q, float* phase, float* lipschitz) {
+  return guarded([&] {
+    require(step && last && q && phase && lipschitz, "bpm_tables: null output");
+    const qh_bpm_options o{obliquity, 0, 0};
+    const BpmTables t = checked_bpm_tables(nz, n, count, illum, geometry, &o, "bpm_tables");
+    std::memcpy(step, t.step.data(), t.step.size() * sizeof(float2));
+    std::memcpy(last, t.last.data(), t.last.size() * sizeof(float2));
+    std::memcpy(q, t.q.data(), t.q.size() * sizeof(float));
+    *phase = t.phase;
+    *lipschitz = t.lipschitz;
+  });
+}
+
+int qh_hip_bpm_forward(const float* x, int nz, int n, const qh_illumination* illum, int count,
+                       const qh_geometry* geometry, const qh_bpm_options* options, qh_c64* out) {
+  return guarded([&] {
+    require(x && out, "bpm_forward: null buffer");
+    const BpmTables t = checked_bpm_tables(nz, n, count, illum, geometry, options, "bpm_forward");
+    require_device();
+    const hipStream_t s = call_stream();
+    BpmModel m(t, options->chunk, options->checkpoint);
+    const std::size_t vol = static_cast<std::size_t>(nz) * n * n;
+    DeviceBuffer<float> dx(vol);
+    DeviceBuffer<float2> dy(static_cast<std::size_t>(count) * n * n);
+    check(hipMemcpyAsync(dx.get(), x, dx.bytes(), hipMemcpyHostToDevice, s), "upload");
+    m.forward(dx.get(), dy.get(), s);
+    check(hipMemcpyAsync(out, dy.get(), dy.bytes(), hipMemcpyDeviceToHost, s), "download");
+    check(hipStreamSynchronize(s), "bpm_forward");
+  });
+}
+
+int qh_hip_bpm_vjp(const float* x, const qh_c64* r, int nz, int n, const qh_illumination* illum,
+                   int count, const qh_geometry* geometry, const qh_bpm_options* options,
+                   float* grad) {
+  return guarded([&] {
+    require(x && r && grad, "bpm_vjp: null buffer");
+    const BpmTables t = checked_bpm_tables(nz, n, count, illum, geometry, options, "bpm_vjp");
+    require_device();
+    const hipStream_t s = call_stream();
+    BpmModel m(t, options->chunk, options->checkpoint);
+    const std::size_t vol = static_cast<std::size_t>(nz) * n * n;
+    DeviceBuffer<float> dx(vol), dg(vol);
+    DeviceBuffer<float2> dr(static_cast<std::size_t>(count) * n * n);
+    check(hipMemcpyAsync(dx.get(), x, dx.bytes(), hipMemcpyHostToDevice, s), "upload");
+    check(hipMemcpyAsync(dr.get(), r, dr.bytes(), hipMemcpyHostToDevice, s), "upload");
+    m.vjp(dx.get(), dr.get(), dg.get(), s);
+    check(hipMemcpyAsync(grad, dg.get(), dg.bytes(), hipMemcpyDeviceToHost, s), "download");
+    check(hipStreamSynchronize(s), "bpm_vjp");
+  });
+}
+
+int qh_hip_bpm_reconstruct(const qh_c64* fields, int count, int n, const qh_illumination* illum,
+                           const qh_geometry* geometry, float* x, int nz,
+                           const qh_bpm_options* options, const qh_bpm_solve_options* solve,
+                           double* loss) {
+  return guarded([&] {
+    require(fields && x && solve && (loss || solve->iterations == 0), "bpm_reconstruct: null buffer");
+    require(solve->iterations >= 0 && solve->step > 0 && solve->tv_weight >= 0 &&
+                solve->tv_inner >= 1 && solve->batch >= 0,
+            "bpm_reconstruct: invalid solver options");
+    require(solve->tv_weight == 0 || n <= 512, "bpm_reconstruct: the TV prox handles n <= 512");
+    const BpmTables t = checked_bpm_tables(nz, n, count, illum, geometry, options, "bpm_reconstruct");
+    require_device();
+    const hipStream_t s = call_stream();
+    BpmModel m(t, options->chunk, options->checkpoint);
+    const std::size_t vol = static_cast<std::size_t>(nz) * n * n;
+    DeviceBuffer<float> dx(vol);
+    DeviceBuffer<float2> df(static_cast<std::size_t>(count) * n * n);
+    check(hipMemcpyAsync(dx.get(), x, dx.bytes(), hipMemcpyHostToDevice, s), "upload");
+    check(hipMemcpyAsync(df.get(), fields, df.bytes(), hipMemcpyHostToDevice, s), "upload");
+    BpmSolveConfig c;
+    c.iterations = solve->iterations;
+    c.batch = solve->batch;
+    c.step = solve->step;
+    c.tv_weight = solve->tv_weight;
+    c.tv_inner = solve->tv_inner;
+    std::vector<double> l(std::max(c.iterations, 1));
+    bpm_solve(m, df.get(), dx.get(), c, l.data(), s);
+    check(hipMemcpyAsync(x, dx.get(), dx.bytes(), hipMemcpyDeviceToHost, s), "download");
+    check(hipStreamSynchronize(s), "bpm_reconstruct");
+    if (c.iterations > 0) std::memcpy(loss, l.data(), c.iterations * sizeof(double));
+  });
+}
+
+int qh_hip_bpm_plan_create(int nz, int n, const qh_illumination* illum, int count,
+                           const qh_geometry* geometry, const qh_bpm_options* options,
+                           qh_bpm_plan** out) {
+  return guarded([&] {
+    require(out != nullptr, "bpm_plan: null output");
+    *out = nullptr;
+    const BpmTables t = checked_bpm_tables(nz, n, count, illum, geometry, options, "bpm_plan");
+    require_device();
+    auto p = std::make_unique<qh_bpm_plan>();
+    p->impl = std::make_unique<BpmModel>(t, options->chunk, options->checkpoint);
+    *out = p.release();
+  });
+}
+
+void qh_hip_bpm_plan_destroy(qh_bpm_plan* plan) { delete plan; }
+
+int qh_hip_bpm_plan_info(const qh_bpm_plan* plan, float* phase, float* lipschitz, int* chunk,
+                         size_t* device_bytes) {
+  return guarded([&] {
+    require(plan != nullptr, "bpm_plan: null plan");
+    if (phase) *phase = plan->impl->phase();
+    if (lipschitz) *lipschitz = plan->impl->lipschitz();
+    if (chunk) *chunk = plan->impl->chunk();
+    if (device_bytes) *device_bytes = plan->impl->bytes();
+  });
+}
+
+namespace {
+hipStream_t plan_stream(void* stream) { return static_cast<hipStream_t>(stream); }
+}  // namespace
+
+int qh_hip_bpm_forward_dev(qh_bpm_plan* plan, const float* x, qh_c64* out, void* stream) {
+  return guarded([&] {
+    require(plan && x && out, "bpm_forward_dev: null argument");
+    plan->impl->forward(x, reinterpret_cast<float2*>(out), plan_stream(stream));
+  });
+}
+
+int qh_hip_bpm_vjp_dev(qh_bpm_plan* plan, const float* x, const qh_c64* r, float* grad,
+                       void* stream) {
+  return guarded([&] {
+    require(plan && x && r && grad, "bpm_vjp_dev: null argument");
+    plan->impl->vjp(x, reinterpret_cast<const float2*>(r), grad, plan_stream(stream));
+  });
+}
+
+int qh_hip_bpm_gradient_dev(qh_bpm_plan* plan, const float* x, const qh_c64* fields, int first,
+                            int last, float* grad, double* loss, void* stream) {
+  return guarded([&] {
+    require(plan && x && fields && grad && loss, "bpm_gradient_dev: null argument");
+    BpmModel& m = *plan->impl;
+    require(first >= 0 && last <= m.count() && first < last,
+            "bpm_gradient_dev: illumination range out of bounds");
+    const hipStream_t s = plan_stream(stream);
+    DeviceBuffer<double> lines(static_cast<std::size_t>(m.count()) * m.n());
+    m.gradient(x, reinterpret_cast<const float2*>(fields), first, last, grad, lines.get(), s);
+    bpm_scale(grad, static_cast<long>(m.nz()) * m.n() * m.n(),
+              1.0f / static_cast<float>(last - first), s);
+    std::vector<double> h(lines.size());
+    check(hipMemcpyAsync(h.data(), lines.get(), lines.bytes(), hipMemcpyDeviceToHost, s), "download");
+    check(hipStreamSynchronize(s), "bpm_gradient_dev");
+    double sum = 0;
+    for (int a = first; a < last; ++a) {
+      double la = 0;
+      for (int y = 0; y < m.n(); ++y) la += h[static_cast<std::size_t>(a) * m.n() + y];
+      sum += la;
+    }
+    *loss = 0.5 * sum / m.count();
   });
 }
 

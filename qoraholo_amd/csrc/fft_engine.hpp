@@ -1,0 +1,232 @@
+// Line engines for fused kernels that transform a batch of lines twice with an elementwise
+// step in between (the split-step beam propagation and its adjoint).
+//
+// A kernel body is written once against this interface and instantiated with either engine:
+//   load(f)          input sample i of global line l is f(l, i)
+//   first(inverse)   transform every line
+//   mid(f)           f(l, k, v&) may read and replace output bin k of line l
+//   reduce_lines(f, out)  out[l] = sum over k of f(l, k, v), summed in a fixed order
+//   second(inverse)  transform every line again
+//   store1(f) / store2(f)  f(l, m, v) receives element m after the first / second transform
+// Lines of a block that lie beyond `lines` are neither read nor written.
+//
+// RegLines: lengths N = N1 N2 (32 .. 512) as register-resident four-step transforms; the first
+// transform leaves thread j with bins N2 k1 + j, which is the input layout of the transform with
+// the factors swapped, so the second transform needs no reordering. LdsLines: any length that
+// factors into 2, 3 and 5 (up to 2048), lines held in LDS between radix passes.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "fft4s.hpp"
+#include "fft_core.hpp"
+
+namespace qh {
+
+// Shape of a batch of lines and the twiddles exp(-2 pi i t / n), t < n.
+struct LineGeom {
+  const float2* tw;
+  FftFactors f;
+  int n;
+  long lines;
+  int lines_per_block;  // LdsLines only
+};
+
+constexpr int ce_max(int a, int b) { return a > b ? a : b; }
+
+template <int N1, int N2, bool LINE_FAST>
+struct RegLines {
+  static constexpr int N = N1 * N2;
+  static constexpr int T = N1 > N2 ? N1 : N2;
+  static constexpr int LINES = 256 / T;
+  static constexpr int kLds =
+      ce_max(t_size<N1, N2, LINES, LINE_FAST>(), t_size<N2, N1, LINES, LINE_FAST>());
+
+  float2* t;
+  float2* tw;
+  int line;
+  int j;
+  long l;
+  bool valid;
+  float2 v[N2];
+  float2 w[N1];
+
+  __device__ static float2* lds_t() {
+    __shared__ float2 s[kLds];
+    return s;
+  }
+  __device__ static float2* lds_tw() {
+    __shared__ float2 s[N];
+    return s;
+  }
+  __device__ static double* lds_red() {
+    __shared__ double s[256];
+    return s;
+  }
+
+  static unsigned grid(long lines) { return static_cast<unsigned>((lines + LINES - 1) / LINES); }
+  static std::size_t dynamic_lds(int) { return 0; }
+
+  __device__ explicit RegLines(const LineGeom& g) {
+    t = lds_t();
+    tw = lds_tw();
+    const int tid = static_cast<int>(threadIdx.x);
+    for (int i = tid; i < N; i += 256) tw[i] = g.tw[i];
+    line = LINE_FAST ? tid % LINES : tid / T;
+    j = LINE_FAST ? tid / LINES : tid % T;
+    l = static_cast<long>(blockIdx.x) * LINES + line;
+    valid = l < g.lines;
+  }
+
+  template <class F>
+  __device__ void load(F f) {
+#pragma unroll
+    for (int n2 = 0; n2 < N2; ++n2) {
+      v[n2] = (valid && j < N1) ? f(l, j + N1 * n2) : float2{0.0f, 0.0f};
+    }
+    __syncthreads();
+  }
+  __device__ void first(bool inverse) {
+    fft_4step<N1, N2, LINES, LINE_FAST>(v, w, j, line, t, tw, inverse);
+  }
+  template <class F>
+  __device__ void mid(F f) {
+    if (valid && j < N2) {
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1) f(l, N2 * k1 + j, w[k1]);
+    }
+  }
+  template <class F>
+  __device__ void reduce_lines(F f, double* out) {
+    double* red = lds_red();
+    double s = 0.0;
+    if (valid && j < N2) {
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1) s += f(l, N2 * k1 + j, w[k1]);
+    }
+    red[line * T + j] = s;
+    __syncthreads();
+    if (valid && j == 0) {
+      double total = 0.0;
+      for (int jj = 0; jj < N2; ++jj) total += red[line * T + jj];
+      out[l] = total;
+    }
+    __syncthreads();
+  }
+  __device__ void second(bool inverse) {
+    fft_4step<N2, N1, LINES, LINE_FAST>(w, v, j, line, t, tw, inverse);
+  }
+  template <class F>
+  __device__ void store1(F f) {
+    if (valid && j < N2) {
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1) f(l, N2 * k1 + j, w[k1]);
+    }
+  }
+  template <class F>
+  __device__ void store2(F f) {
+    if (valid && j < N1) {
+#pragma unroll
+      for (int k1 = 0; k1 < N2; ++k1) f(l, N1 * k1 + j, v[k1]);
+    }
+  }
+};
+
+// CONTIG: lines contiguous in memory, threads walk along lines. Otherwise lines are adjacent
+// columns: threads walk across lines so that each row of the block is one coalesced segment.
+template <bool CONTIG>
+struct LdsLines {
+  float2* tw;
+  float2* b0;
+  float2* b1;
+  float2* cur;
+  const FftFactors* f;
+  int n;
+  int nl;
+  int ls;
+  int es;
+  long l0;
+
+  static int lines_per_block(int n) {
+    return CONTIG ? (2048 / n > 64 ? 64 : (2048 / n < 1 ? 1 : 2048 / n))
+                  : (2048 / n > 16 ? 16 : (2048 / n < 1 ? 1 : 2048 / n));
+  }
+  static unsigned grid(long lines, int lpb) { return static_cast<unsigned>((lines + lpb - 1) / lpb); }
+  static std::size_t dynamic_lds(int n) {
+    return (static_cast<std::size_t>(n) + 2 * static_cast<std::size_t>(lines_per_block(n)) * n) *
+           sizeof(float2);
+  }
+
+  __device__ explicit LdsLines(const LineGeom& g) {
+    extern __shared__ float2 smem[];
+    n = g.n;
+    f = &g.f;
+    tw = smem;
+    b0 = smem + n;
+    b1 = b0 + g.lines_per_block * n;
+    cur = b0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = g.tw[i];
+    l0 = static_cast<long>(blockIdx.x) * g.lines_per_block;
+    const long left = g.lines - l0;
+    nl = static_cast<int>(left < g.lines_per_block ? left : g.lines_per_block);
+    ls = CONTIG ? n : 1;
+    es = CONTIG ? 1 : nl;
+  }
+
+  __device__ void split(int e, int& line, int& i) const {
+    if (CONTIG) {
+      line = e / n;
+      i = e - line * n;
+    } else {
+      i = e / nl;
+      line = e - i * nl;
+    }
+  }
+
+  template <class F>
+  __device__ void load(F fn) {
+    for (int e = threadIdx.x; e < nl * n; e += blockDim.x) {
+      int line, i;
+      split(e, line, i);
+      b0[line * ls + i * es] = fn(l0 + line, i);
+    }
+    cur = b0;
+    __syncthreads();
+  }
+  __device__ void first(bool inverse) { cur = lds_fft(b0, b1, ls, es, nl, *f, tw, inverse, !CONTIG); }
+  template <class F>
+  __device__ void mid(F fn) {
+    for (int e = threadIdx.x; e < nl * n; e += blockDim.x) {
+      int line, k;
+      split(e, line, k);
+      fn(l0 + line, k, cur[line * ls + k * es]);
+    }
+    __syncthreads();
+  }
+  template <class F>
+  __device__ void reduce_lines(F fn, double* out) {
+    if (static_cast<int>(threadIdx.x) < nl) {
+      const int line = static_cast<int>(threadIdx.x);
+      double s = 0.0;
+      for (int k = 0; k < n; ++k) s += fn(l0 + line, k, cur[line * ls + k * es]);
+      out[l0 + line] = s;
+    }
+    __syncthreads();
+  }
+  __device__ void second(bool inverse) {
+    float2* other = cur == b0 ? b1 : b0;
+    cur = lds_fft(cur, other, ls, es, nl, *f, tw, inverse, !CONTIG);
+  }
+  template <class F>
+  __device__ void store1(F fn) {
+    for (int e = threadIdx.x; e < nl * n; e += blockDim.x) {
+      int line, m;
+      split(e, line, m);
+      fn(l0 + line, m, cur[line * ls + m * es]);
+    }
+  }
+  template <class F>
+  __device__ void store2(F fn) { store1(fn); }
+};
+
+}  // namespace qh

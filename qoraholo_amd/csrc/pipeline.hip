@@ -126,7 +126,6 @@ void Pipeline::process() {
 
 void Pipeline::process_device(const float* holo_dev, float* ri_dev, hipStream_t s) {
   if (!has_background_) throw LogicError("Pipeline: set_background() first");
-  if (s == nullptr) s = compute_.get();
   front_end(holo_dev, s, false);
   solver_.run(fields_.get(), s, &clock_, kSolver);
   const std::size_t vol = static_cast<std::size_t>(solver_.nz()) * solver_.n() * solver_.n();

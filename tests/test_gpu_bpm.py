@@ -1,0 +1,210 @@
+"""Beam propagation on the GPU against the oracle: forward model, adjoint (vjp) with slice
+checkpointing, gradient and loss, the inversion, the torch.autograd.Function, and invariants
+at the full 256^2 x 256 / 49-illumination shape (unit envelope, checkpoint and chunk
+invariance bit for bit, linearity of the adjoint)."""
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+GEOM = (0.532, 1.337, 1.2, 0.2)
+
+
+def spheres(ph):
+    """The two nested spheres of the reference's tests/test_cuda.cpp:30-31."""
+    return [ph.Sphere(0.0, 0.4, -0.3, 2.5, 1.36), ph.Sphere(0.0, 0.4, -0.3, 1.0, 1.38)]
+
+
+def residual_pattern(count, n):
+    i = np.arange(count * n * n, dtype=np.float64)
+    return (np.sin(0.7 * i) + 1j * np.cos(1.3 * i)).astype(np.complex64).reshape(count, n, n)
+
+
+def case(qh, nz, n, count=7, na=0.9):
+    from qoraholo_amd import phantom as ph
+
+    g = qh.Geometry(*GEOM)
+    il = qh.circular_scan(count, na, g)
+    ri = ph.render_phantom(spheres(ph), g, nz, n)
+    return g, il, ri, (ri - np.float32(GEOM[1])).astype(np.float32)
+
+
+@pytest.mark.parametrize("nz,n", [(32, 48), (16, 32), (24, 64), (12, 128), (6, 256), (4, 512),
+                                  (10, 60), (8, 192)])
+def test_forward_matches_oracle(device, qh, oracle, nz, n):
+    """Scattered part y - 1 against the FP64 oracle: as close as the float oracle (the
+    reference's float algorithm) is, within a factor 2. At large n with a small object the
+    scattered part is small against 1 and float rounding of y sets the floor for both."""
+    g, il, ri, x = case(qh, nz, n)
+    print(f"bpm forward nz={nz} n={n}", flush=True)
+    for obliquity in (True, False):
+        got = qh.bpm_forward(ri, il, g, obliquity=obliquity)
+        f32 = oracle.bpm_forward(x, il, GEOM, obliquity=obliquity)
+        f64 = oracle.bpm_forward(x, il, GEOM, obliquity=obliquity, dtype=np.float64)
+        assert got.shape == f64.shape
+        e_ref = rel_l2(f32 - 1, f64 - 1)
+        e_gpu = rel_l2(got - 1, f64 - 1)
+        assert e_gpu < max(2 * e_ref, 1e-5), (obliquity, e_gpu, e_ref)
+        assert rel_l2(got - 1, f32 - 1) < 5e-5
+
+
+def test_forward_chunks_are_bitwise_equal(device, qh):
+    g, il, ri, _ = case(qh, 32, 48)
+    a = qh.bpm_forward(ri, il, g)
+    for chunk in (1, 3):
+        assert np.array_equal(qh.bpm_forward(ri, il, g, chunk=chunk), a)
+
+
+@pytest.mark.parametrize("nz,n", [(32, 48), (16, 32), (20, 64)])
+def test_vjp_matches_oracle(device, qh, oracle, nz, n):
+    g, il, _, x = case(qh, nz, n)
+    r = residual_pattern(il.shape[0], n)
+    print(f"bpm vjp nz={nz} n={n}", flush=True)
+    got = qh.bpm_vjp(x, r, il, g)
+    want = oracle.bpm_vjp(x, r, il, GEOM)
+    # The reference's CUDA-vs-CPU bound is 1e-4 (tests/test_cuda.cpp:149-160).
+    assert rel_l2(got, want) < 2e-5
+
+
+def test_vjp_invariant_to_checkpoints_and_chunks(device, qh):
+    """Recomputed envelopes are the stored ones bit for bit, and the gradient terms are summed
+    in illumination order whatever the chunking."""
+    g, il, _, x = case(qh, 32, 48)
+    r = residual_pattern(il.shape[0], 48)
+    ref = qh.bpm_vjp(x, r, il, g, checkpoint=0)
+    for checkpoint, chunk in ((16, 0), (5, 0), (1, 0), (0, 2), (7, 3)):
+        assert np.array_equal(qh.bpm_vjp(x, r, il, g, checkpoint=checkpoint, chunk=chunk), ref), (
+            checkpoint, chunk)
+
+
+def test_adjoint_identity_on_device(device, qh, oracle):
+    """<J h, r> with the FP64 tangent model against <h, J^T r> from the device adjoint."""
+    g, il, _, x = case(qh, 16, 32, count=5)
+    rng = np.random.default_rng(3)
+    h = rng.standard_normal(x.shape).astype(np.float32)
+    r = residual_pattern(il.shape[0], 32)
+    jh = oracle.bpm_jvp(x, h, il, GEOM, dtype=np.float64)
+    lhs = np.sum(np.real(np.conj(jh) * r.astype(np.complex128)))
+    rhs = np.sum(h.astype(np.float64) * qh.bpm_vjp(x, r, il, g).astype(np.float64))
+    assert abs(lhs - rhs) / abs(rhs) < 2e-5
+
+
+def test_gradient_against_fp64_differences(device, qh, oracle):
+    """Central differences (step 1e-4) of the FP64 oracle's loss at 32 seeded voxels against
+    the device gradient."""
+    torch = pytest.importorskip("torch")
+    g, il, _, x = case(qh, 16, 32, count=5)
+    data = oracle.bpm_forward((x * 1.2).astype(np.float32), il, GEOM)
+    plan = qh.BpmPlan(16, 32, il, g)
+    xt = torch.from_numpy(x).cuda()
+    dt = torch.from_numpy(data).cuda()
+    grad, loss = plan.gradient(xt, dt)
+    grad = grad.cpu().numpy()
+    x64 = x.astype(np.float64)
+    _, l64 = oracle.bpm_gradient(x64, data, il, GEOM, dtype=np.float64)
+    assert loss == pytest.approx(l64, rel=1e-5)
+    rng = np.random.default_rng(5)
+    eps = 1e-4
+    for i in rng.choice(x.size, 32, replace=False):
+        xp = x64.copy().reshape(-1)
+        xm = x64.copy().reshape(-1)
+        xp[i] += eps
+        xm[i] -= eps
+        _, lp = oracle.bpm_gradient(xp.reshape(x.shape), data, il, GEOM, dtype=np.float64)
+        _, lm = oracle.bpm_gradient(xm.reshape(x.shape), data, il, GEOM, dtype=np.float64)
+        fd = (lp - lm) / (2 * eps)
+        assert grad.reshape(-1)[i] == pytest.approx(fd, rel=2e-3, abs=1e-4 * np.abs(grad).max())
+
+
+def test_plan_gradient_matches_oracle(device, qh, oracle):
+    torch = pytest.importorskip("torch")
+    g, il, _, x = case(qh, 32, 48)
+    data = oracle.bpm_forward((x * 0.8).astype(np.float32), il, GEOM)
+    plan = qh.BpmPlan(32, 48, il, g)
+    xt = torch.from_numpy(x).cuda()
+    dt = torch.from_numpy(data).cuda()
+    for first, last in ((0, 7), (2, 5)):
+        grad, loss = plan.gradient(xt, dt, first, last)
+        want, wl = oracle.bpm_gradient(x, data, il, GEOM, first=first, last=last)
+        assert rel_l2(grad.cpu().numpy(), want) < 2e-5
+        assert loss == pytest.approx(wl, rel=2e-5)
+
+
+def test_autograd_function(device, qh):
+    """torch.autograd through the plan: the backward of 0.5 sum |bpm(x) - d|^2 is the adjoint
+    of the residual, count times the plan's gradient (which carries 1 / count)."""
+    torch = pytest.importorskip("torch")
+    g, il, _, x = case(qh, 16, 64)
+    plan = qh.BpmPlan(16, 64, il, g)
+    xt = torch.from_numpy(x).cuda()
+    d = plan.forward(xt * 0.9)
+    xr = xt.clone().requires_grad_(True)
+    y = qh.bpm(xr, plan)
+    loss = 0.5 * torch.sum(torch.abs(y - d) ** 2)
+    loss.backward()
+    grad, l2 = plan.gradient(xt, d)
+    count = il.shape[0]
+    assert rel_l2(xr.grad.cpu().numpy(), count * grad.cpu().numpy()) < 1e-5
+    assert loss.item() / count == pytest.approx(l2, rel=1e-5)
+    direct = plan.vjp(xt, (y - d).detach().contiguous())
+    assert rel_l2(direct.cpu().numpy(), xr.grad.cpu().numpy()) < 1e-6
+
+
+@pytest.mark.parametrize("batch,tv", [(0, 0.0), (3, 0.0), (0, 1e-2), (3, 1e-2)])
+def test_reconstruct_matches_oracle(device, qh, oracle, batch, tv):
+    """tests/test_cuda.cpp:163-183 (5 iterations from a flat start, with and without batches and
+    TV); the reference's bound is 1e-3 on the contrast and on the last loss."""
+    g, il, ri, x = case(qh, 32, 48)
+    data = oracle.bpm_forward(x, il, GEOM)
+    start = np.full_like(ri, np.float32(GEOM[1]))
+    print(f"bpm reconstruct batch={batch} tv={tv}", flush=True)
+    got, gl = qh.bpm_reconstruct(data, il, g, start, iterations=5, batch=batch, tv_weight=tv,
+                                 return_loss=True)
+    want, wl = oracle.bpm_reconstruct(data, il, GEOM, start, iterations=5, batch=batch,
+                                      tv_weight=tv)
+    nm = np.float32(GEOM[1])
+    assert rel_l2(got - nm, want - nm) < 1e-4
+    assert np.allclose(gl, wl, rtol=1e-4)
+    assert gl[-1] < gl[0]
+
+
+def test_reconstruct_chunk_invariant(device, qh, oracle):
+    """The gradient computed in device passes of 2 illuminations gives the same result
+    (tests/test_cuda.cpp:186-196; here bit for bit)."""
+    g, il, ri, x = case(qh, 32, 48)
+    data = oracle.bpm_forward(x, il, GEOM)
+    start = np.full_like(ri, np.float32(GEOM[1]))
+    a = qh.bpm_reconstruct(data, il, g, start, iterations=3)
+    b = qh.bpm_reconstruct(data, il, g, start, iterations=3, chunk=2, checkpoint=8)
+    assert np.array_equal(a, b)
+
+
+def test_full_size_invariants(device, qh):
+    """256^2 x 256 with 49 illuminations: the unit envelope passes through x = 0 exactly,
+    checkpointed and stored envelopes give the same adjoint bit for bit, and the adjoint is
+    linear in r."""
+    torch = pytest.importorskip("torch")
+    g = qh.Geometry(*GEOM)
+    il = qh.circular_scan(49, 0.9, g)
+    n = nz = 256
+    print("bpm full size 256^3 x 49", flush=True)
+    plan = qh.BpmPlan(nz, n, il, g)
+    zero = torch.zeros((nz, n, n), dtype=torch.float32, device="cuda")
+    y0 = plan.forward(zero)
+    assert torch.equal(y0, torch.ones_like(y0))
+    gen = torch.Generator(device="cuda").manual_seed(7)
+    x = 0.02 * torch.rand((nz, n, n), generator=gen, device="cuda")
+    r1 = torch.randn((49, n, n), dtype=torch.complex64, generator=gen, device="cuda")
+    r2 = torch.randn((49, n, n), dtype=torch.complex64, generator=gen, device="cuda")
+    g1 = plan.vjp(x, r1)
+    g2 = plan.vjp(x, r2)
+    g12 = plan.vjp(x, (r1 + r2).contiguous())
+    lin = float(torch.linalg.vector_norm(g12 - g1 - g2) / torch.linalg.vector_norm(g12))
+    assert lin < 1e-5
+    stored = qh.BpmPlan(nz, n, il, g, checkpoint=0)
+    assert torch.equal(stored.vjp(x, r1), g1)
+    del stored
+    torch.cuda.synchronize()

@@ -124,3 +124,47 @@ def test_cell_shapes_are_nested(qh):
                 assert (p[0] / body.az) ** 2 + (p[1] / body.ay) ** 2 + (p[2] / body.ax) ** 2 < 1
     for a, b in ((nucleus, v1), (nucleus, v2), (v1, v2)):
         assert math.dist((a.z, a.y, a.x), (b.z, b.y, b.x)) > a.radius + b.radius
+
+
+def test_bpm_tables_equal_oracle_bitwise(oracle, qh):
+    """The library's tables (src/bpm.cpp:10-60, double on the host) are the oracle's bits."""
+    g = qh.Geometry(0.532, 1.337, 1.2, 0.25)
+    il = qh.circular_scan(5, 0.8, g)
+    for n, nz, obliquity in ((48, 32, True), (32, 16, False), (64, 7, True)):
+        got = qh.bpm_tables(n, nz, il, g, obliquity)
+        want = oracle.bpm_tables(n, nz, il, (0.532, 1.337, 1.2, 0.25), obliquity)
+        for a, b in zip(got[:3], want[:3]):
+            assert np.array_equal(a, b)
+        assert got[3] == want[3] and got[4] == want[4]
+
+
+def test_bpm_argument_checks(qh):
+    """src/api.cpp:141-157,180-188: rejected before any device work, as ValueError."""
+    g = qh.Geometry(0.532, 1.337, 1.2, 0.2)
+    il = qh.circular_scan(3, 0.9, g)
+    ri = np.full((4, 32, 32), 1.337, np.float32)
+    with pytest.raises(ValueError):
+        qh.bpm_forward(np.full((4, 31, 31), 1.337, np.float32), il, g)
+    with pytest.raises(ValueError):
+        qh.bpm_forward(np.full((4, 32, 16), 1.337, np.float32), il, g)
+    with pytest.raises(ValueError):
+        qh.bpm_forward(ri, np.array([[3.0, 0.0]], np.float32), g)
+    with pytest.raises(ValueError):
+        qh.bpm_forward(ri, il, qh.Geometry(0.532, 1.337, 1.2, 0.0))
+    with pytest.raises(ValueError):
+        qh.bpm_forward(np.full((4, 28, 28), 1.337, np.float32), il, g)  # 28 = 4 * 7
+    with pytest.raises(ValueError):
+        qh.bpm_forward(ri, il, g, backend="cuda")
+    fields = np.ones((3, 32, 32), np.complex64)
+    with pytest.raises(ValueError):
+        qh.bpm_reconstruct(fields, il, g, ri, step=0.0)
+    with pytest.raises(ValueError):
+        qh.bpm_reconstruct(fields, il, g, ri, tv_inner=0)
+    with pytest.raises(ValueError):
+        qh.bpm_reconstruct(fields, il, g, ri, iterations=-1)
+    with pytest.raises(ValueError):
+        qh.bpm_reconstruct(fields[:, :16, :16], il, g, ri)
+    with pytest.raises(ValueError):
+        qh.bpm_reconstruct(fields[:2], il, g, ri)
+    with pytest.raises(ValueError):
+        qh.bpm_vjp(ri - np.float32(1.337), fields[:2], il, g)
