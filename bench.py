@@ -93,6 +93,105 @@ def fgp_traffic_run(qh, torch) -> int:
     return 0
 
 
+BPM_N = 256
+BPM_ANGLES = 49
+BPM_GEOM = (0.532, 1.337, 1.2, 0.2)
+
+
+def bpm_leg(qh, torch, cpu=True, repeats=5):
+    """Beam propagation at 256^2 x 256 with 49 illuminations (NA 0.9): forward model, data-term
+    gradient (forward, residual, checkpointed adjoint) and one inversion iteration, on three
+    overlapping beads (Delta n <= 0.05). Inputs and outputs of the timed calls stay in HBM
+    except for the inversion, which runs from host arrays like the reference's bpm_reconstruct.
+    The CPU baseline is the oracle's forward on as many of the 49 illuminations as there are
+    threads (one illumination per thread), scaled to 49."""
+    from qoraholo_amd import phantom as ph
+
+    g = qh.Geometry(*BPM_GEOM)
+    il = qh.circular_scan(BPM_ANGLES, 0.9, g)
+    beads = [ph.Sphere(0.0, 0.0, 0.0, 4.0, 1.385), ph.Sphere(2.5, 1.5, -1.0, 3.0, 1.372),
+             ph.Sphere(-2.0, -2.0, 2.0, 2.0, 1.360)]
+    ri = ph.render_phantom(beads, g, BPM_N, BPM_N)
+    x_true = (ri - np.float32(g.n_medium)).astype(np.float32)
+    dev = torch.device("cuda:0")
+    plan = qh.BpmPlan(BPM_N, BPM_N, il, g)
+    xt = torch.from_numpy(x_true).to(dev)
+    data = plan.forward(xt)
+    x0 = torch.zeros_like(xt)
+    torch.cuda.synchronize()
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(repeats):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / repeats
+
+    y = torch.empty_like(data)
+    grad = torch.empty_like(xt)
+    fwd_ms = timed(lambda: plan.forward(xt, out=y))
+    grad_ms = timed(lambda: plan.gradient(x0, data, out=grad))
+    log(f"bpm: forward {fwd_ms:.2f} ms, gradient {grad_ms:.2f} ms")
+
+    fields = data.cpu().numpy()
+    start = np.full_like(ri, np.float32(g.n_medium))
+    t0 = time.perf_counter()
+    qh.bpm_reconstruct(fields, il, g, start, iterations=1)
+    t1 = time.perf_counter()
+    _, loss = qh.bpm_reconstruct(fields, il, g, start, iterations=6, return_loss=True)
+    t6 = time.perf_counter()
+    iter_ms = 1000.0 * ((t6 - t1) - (t1 - t0)) / 5.0
+    log(f"bpm: one inversion iteration {iter_ms:.2f} ms (host arrays, from 1 and 6 iterations)")
+
+    a, plane, nz = BPM_ANGLES, BPM_N * BPM_N, BPM_N
+    # The reference's traffic model (88 bytes per pixel per slice per illumination for the
+    # forward; 222 GB for a gradient with recomputation from checkpoints every 16 slices).
+    fwd_alg = 88 * plane * a * nz
+    grad_alg = int(fwd_alg * (1 + 15 / 16) + 100 * plane * a * nz)
+    out = {
+        "workload": "3 overlapping beads, 256^2 x 256 at 0.2 um, 49 illuminations NA 0.9, "
+                    "checkpoint every 16 slices",
+        "forward_ms": round(fwd_ms, 3),
+        "gradient_ms": round(grad_ms, 3),
+        "iteration_ms_host_arrays": round(iter_ms, 3),
+        "device_bytes": plan.device_bytes,
+        "forward_roofline": {"bound": "hbm", "algorithmic_bytes": fwd_alg,
+                             "achieved": round(fwd_alg / (fwd_ms * 1e-3) / 1e9, 1),
+                             "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": round(fwd_alg / (fwd_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "gradient_roofline": {"bound": "hbm", "algorithmic_bytes": grad_alg,
+                              "achieved": round(grad_alg / (grad_ms * 1e-3) / 1e9, 1),
+                              "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": round(grad_alg / (grad_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "loss_6_iterations": [float(v) for v in loss],
+        "reference_rtx5070_ms": {"forward_host_in_out": 130, "iteration": 170},
+    }
+    if cpu:
+        import oracle
+
+        cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+        sample = max(1, min(a, cores))
+        il7 = il[:sample]
+        t0 = time.perf_counter()
+        want = oracle.bpm_forward(x_true, il7, BPM_GEOM)
+        secs = time.perf_counter() - t0
+        got = y[:sample].cpu().numpy()
+        err = float(np.sqrt(np.sum(np.abs(got.astype(np.complex128) - want) ** 2) /
+                            np.sum(np.abs(want.astype(np.complex128) - 1) ** 2)))
+        out["cpu_baseline"] = {"value": round(secs * 1000.0 * a / sample, 1), "unit": "ms/forward",
+                               "cores": cores, "kind": "port",
+                               "sample": f"oracle forward on {sample} of {a} illuminations, "
+                                         f"{secs:.2f} s, scaled to {a}"}
+        out["parity"] = {"scattered_rel_l2_vs_oracle": err, "illuminations": sample}
+        log(f"bpm: oracle forward on {sample} illuminations {secs:.2f} s; parity {err:.2e}")
+    return out
+
+
 def pct(v, q):
     return round(float(np.percentile(v, q)), 3)
 
@@ -195,6 +294,9 @@ def main() -> int:
                     help="skip the oracle run (CPU baseline and full-size parity)")
     ap.add_argument("--no-serving", action="store_true",
                     help="skip the served-versus-plain comparison (host-memory requests)")
+    ap.add_argument("--no-bpm", action="store_true", help="skip the beam propagation leg")
+    ap.add_argument("--bpm-only", action="store_true",
+                    help="run only the beam propagation leg and print its JSON object")
     ap.add_argument("--fgp-traffic", action="store_true",
                     help="run only 5 FGP steps at 256^3 and a 256 MiB device copy, for the "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE runs that calibrate HBM traffic")
@@ -215,6 +317,10 @@ def main() -> int:
 
     if not qh.hip_available():
         raise SystemExit("no HIP device or libqoraholo_hip.so missing")
+    if args.bpm_only:
+        print(json.dumps(bpm_leg(qh, torch, cpu=not args.no_cpu_baseline, repeats=args.steps)),
+              flush=True)
+        return 0
     t0 = time.perf_counter()
     w = workloads.readme_cell(n=N, angles=ANGLES, seed=0)
     log(f"workload generated in {time.perf_counter() - t0:.1f} s")
@@ -304,6 +410,9 @@ def main() -> int:
                                          "p99": serving["plain"]["p99_ms"]}
         result["latency_ms"]["served"] = {"p50": serving["served_poisson"]["p50_ms"],
                                           "p99": serving["served_poisson"]["p99_ms"]}
+
+    if not args.no_bpm:
+        result["bpm"] = bpm_leg(qh, torch, cpu=not args.no_cpu_baseline)
 
     if not args.no_cpu_baseline:
         result["cpu_baseline"], want = cpu_baseline(w)

@@ -108,7 +108,7 @@ using Args = BpmModel::Args;
 // forward x transform. in == null: the unit envelope of the first slice. out == null: store the
 // envelope only.
 template <class E>
-__global__ void __launch_bounds__(256) k_bpm_rows_fwd(Args a) {
+__global__ void __launch_bounds__(E::kBlock) k_bpm_rows_fwd(Args a) {
   E e(a.g);
   const int n = a.g.n;
   if (a.in == nullptr) {
@@ -140,7 +140,7 @@ __device__ __forceinline__ long col_at(long l, int i, int n, long plane) {
 // Columns (line l = b n + kx, element i at b n^2 + i n + kx): forward y transform, propagation
 // step S = c (q A + (1 - q) S) (src/common.hpp:158-163), inverse y transform.
 template <class E>
-__global__ void __launch_bounds__(256) k_bpm_cols_fwd(Args a) {
+__global__ void __launch_bounds__(E::kBlock) k_bpm_cols_fwd(Args a) {
   E e(a.g);
   const int n = a.g.n;
   const long plane = static_cast<long>(n) * n;
@@ -168,7 +168,7 @@ __global__ void __launch_bounds__(256) k_bpm_cols_fwd(Args a) {
 // with data, r = y - d, the per-row data term, and the forward x transform of r, the first
 // half of the transform that starts the adjoint.
 template <class E>
-__global__ void __launch_bounds__(256) k_bpm_rows_out(Args a) {
+__global__ void __launch_bounds__(E::kBlock) k_bpm_rows_out(Args a) {
   E e(a.g);
   const int n = a.g.n;
   e.load([&](long l, int i) { return a.in[l * n + i]; });
@@ -197,7 +197,7 @@ __global__ void __launch_bounds__(256) k_bpm_rows_out(Args a) {
 // FFT2(r) on the first slice processed); with the carried g, U = conj(c) (g + back), g <- (1-q) U,
 // b = q U (src/common.hpp:168-176); inverse y transform of b.
 template <class E>
-__global__ void __launch_bounds__(256) k_bpm_cols_adj(Args a) {
+__global__ void __launch_bounds__(E::kBlock) k_bpm_cols_adj(Args a) {
   E e(a.g);
   const int n = a.g.n;
   const long plane = static_cast<long>(n) * n;
@@ -226,7 +226,7 @@ __global__ void __launch_bounds__(256) k_bpm_cols_adj(Args a) {
 // Adjoint rows: b = inverse x transform / n^2; gradient term phase Im(conj(t w) b) per
 // illumination; back = conj(t) b and its forward x transform (skipped on slice 0).
 template <class E>
-__global__ void __launch_bounds__(256) k_bpm_rows_adj(Args a) {
+__global__ void __launch_bounds__(E::kBlock) k_bpm_rows_adj(Args a) {
   E e(a.g);
   const int n = a.g.n;
   e.load([&](long l, int i) { return a.in[l * n + i]; });
@@ -243,13 +243,22 @@ __global__ void __launch_bounds__(256) k_bpm_rows_adj(Args a) {
   e.store2([&](long l, int m, float2 v) { a.out[l * n + m] = v; });
 }
 
-// grad[i] += p[b][i] for b = 0 .. chunk-1 in order.
+// grad[i] += p[b][i] for b = 0 .. chunk-1 in order; the loads of eight illuminations are
+// issued together so that each thread keeps several in flight.
 __global__ void __launch_bounds__(256) k_bpm_reduce(const float* p, int chunk, long plane,
                                                     float* grad) {
   const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= plane) return;
   float g = grad[i];
-  for (int b = 0; b < chunk; ++b) g += p[b * plane + i];
+  int b = 0;
+  for (; b + 8 <= chunk; b += 8) {
+    float v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = p[(b + u) * plane + i];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) g += v[u];
+  }
+  for (; b < chunk; ++b) g += p[b * plane + i];
   grad[i] = g;
 }
 
@@ -303,21 +312,21 @@ template <bool ROWS, class E>
 void launch_with(Kind k, Args a, unsigned grid, std::size_t lds, hipStream_t s) {
   if constexpr (ROWS) {
     switch (k) {
-      case Kind::rows_fwd: hipLaunchKernelGGL(k_bpm_rows_fwd<E>, dim3(grid), dim3(256), lds, s, a); break;
-      case Kind::rows_out: hipLaunchKernelGGL(k_bpm_rows_out<E>, dim3(grid), dim3(256), lds, s, a); break;
-      default: hipLaunchKernelGGL(k_bpm_rows_adj<E>, dim3(grid), dim3(256), lds, s, a); break;
+      case Kind::rows_fwd: hipLaunchKernelGGL(k_bpm_rows_fwd<E>, dim3(grid), dim3(E::kBlock), lds, s, a); break;
+      case Kind::rows_out: hipLaunchKernelGGL(k_bpm_rows_out<E>, dim3(grid), dim3(E::kBlock), lds, s, a); break;
+      default: hipLaunchKernelGGL(k_bpm_rows_adj<E>, dim3(grid), dim3(E::kBlock), lds, s, a); break;
     }
   } else {
     switch (k) {
-      case Kind::cols_fwd: hipLaunchKernelGGL(k_bpm_cols_fwd<E>, dim3(grid), dim3(256), lds, s, a); break;
-      default: hipLaunchKernelGGL(k_bpm_cols_adj<E>, dim3(grid), dim3(256), lds, s, a); break;
+      case Kind::cols_fwd: hipLaunchKernelGGL(k_bpm_cols_fwd<E>, dim3(grid), dim3(E::kBlock), lds, s, a); break;
+      default: hipLaunchKernelGGL(k_bpm_cols_adj<E>, dim3(grid), dim3(E::kBlock), lds, s, a); break;
     }
   }
 }
 
 template <bool ROWS, int N1, int N2>
 void launch_reg(Kind k, Args a, hipStream_t s) {
-  using E = RegLines<N1, N2, !ROWS>;
+  using E = RegLines<N1, N2, !ROWS, ROWS ? 64 : 256>;
   a.g.lines_per_block = E::LINES;
   launch_with<ROWS, E>(k, a, E::grid(a.g.lines), 0, s);
 }

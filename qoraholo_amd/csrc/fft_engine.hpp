@@ -12,8 +12,11 @@
 //
 // RegLines: lengths N = N1 N2 (32 .. 512) as register-resident four-step transforms; the first
 // transform leaves thread j with bins N2 k1 + j, which is the input layout of the transform with
-// the factors swapped, so the second transform needs no reordering. LdsLines: any length that
-// factors into 2, 3 and 5 (up to 2048), lines held in LDS between radix passes.
+// the factors swapped, so the second transform needs no reordering. BLOCK threads hold
+// BLOCK / max(N1, N2) lines: one-wave blocks suit contiguous rows (many small blocks in flight,
+// cheap barriers); columns keep 256 threads so that each load spans whole 128-byte lines.
+// LdsLines: any length that factors into 2, 3 and 5 (up to 2048), lines held in LDS between
+// radix passes, 256 threads.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -34,11 +37,13 @@ struct LineGeom {
 
 constexpr int ce_max(int a, int b) { return a > b ? a : b; }
 
-template <int N1, int N2, bool LINE_FAST>
+template <int N1, int N2, bool LINE_FAST, int BLOCK = 256>
 struct RegLines {
+  static constexpr int kBlock = BLOCK;
   static constexpr int N = N1 * N2;
   static constexpr int T = N1 > N2 ? N1 : N2;
-  static constexpr int LINES = 256 / T;
+  static constexpr int LINES = BLOCK / T;
+  static_assert(LINES >= 1, "a block must hold at least one line");
   static constexpr int kLds =
       ce_max(t_size<N1, N2, LINES, LINE_FAST>(), t_size<N2, N1, LINES, LINE_FAST>());
 
@@ -60,7 +65,7 @@ struct RegLines {
     return s;
   }
   __device__ static double* lds_red() {
-    __shared__ double s[256];
+    __shared__ double s[BLOCK];
     return s;
   }
 
@@ -71,7 +76,7 @@ struct RegLines {
     t = lds_t();
     tw = lds_tw();
     const int tid = static_cast<int>(threadIdx.x);
-    for (int i = tid; i < N; i += 256) tw[i] = g.tw[i];
+    for (int i = tid; i < N; i += BLOCK) tw[i] = g.tw[i];
     line = LINE_FAST ? tid % LINES : tid / T;
     j = LINE_FAST ? tid / LINES : tid % T;
     l = static_cast<long>(blockIdx.x) * LINES + line;
@@ -136,6 +141,7 @@ struct RegLines {
 // columns: threads walk across lines so that each row of the block is one coalesced segment.
 template <bool CONTIG>
 struct LdsLines {
+  static constexpr int kBlock = 256;
   float2* tw;
   float2* b0;
   float2* b1;
