@@ -165,6 +165,20 @@ int qh_hip_map_kspace(const qh_c64* fields, int count, int n, const qh_illuminat
                       const qh_odt_grid* grid, qh_c64* potential, uint16_t* counts,
                       double* filled);
 
+/* Sideband of every hologram of a stack: find_sideband (src/cpu/field.cpp:35-60) on the device.
+ * The reference runs it on the host for the first hologram only; here the spectra and the peak
+ * search run on the device and the sub-bin refinement (field.cpp:24-33) on the host.
+ * dc_exclusion <= 0 means n / 8. out: count x 5 floats (iy, ix, y, x, magnitude). */
+int qh_hip_find_sidebands(const float* holograms, int count, int n, float dc_exclusion,
+                          float* out);
+/* Illumination frequencies estimated from background holograms as examples/readme_figure.py:39-46
+ * does: the carrier c is the sideband of background 0 and f_a = (s_a - c) / (n pixel), with the
+ * hologram pixel. carrier receives c as 5 floats (iy, ix, y, x, magnitude). The cold-start step
+ * of a new geometry in the served loop. */
+int qh_hip_estimate_illuminations(const float* backgrounds, int count, int n,
+                                  float hologram_pixel, float dc_exclusion, qh_illumination* out,
+                                  float* carrier);
+
 /* ---- Pipeline (include/qoraholo/pipeline.hpp:50-85; src/backends.hpp:75-89,110-111) ---- */
 
 typedef struct qh_pipeline qh_pipeline;

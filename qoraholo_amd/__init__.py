@@ -45,7 +45,9 @@ __all__ = [
     "cuda_available",
     "extract_fields",
     "field_ratio",
+    "estimate_illuminations",
     "find_sideband",
+    "find_sidebands",
     "forward_fields",
     "hip_available",
     "kspace",
@@ -171,6 +173,39 @@ def find_sideband(hologram, dc_exclusion: float = 0.0) -> Sideband:
     rx = refine(s[y, x - 1], s[y, x], s[y, x + 1])
     return Sideband(int(y - c), int(x - c), float(np.float32(y - c) + ry),
                     float(np.float32(x - c) + rx), best)
+
+
+def find_sidebands(holograms, dc_exclusion: float = 0.0) -> list[Sideband]:
+    """The sideband of every hologram of a stack, found on the device.
+
+    ``find_sideband`` (src/cpu/field.cpp:35-60) per image: spectra and peak search run on the
+    device, the sub-bin refinement on the host; the reference runs it on the host for one
+    hologram at a time.
+    """
+    h, _ = _stack(holograms, np.float32)
+    count, ny, n = h.shape
+    if ny != n or n % 2 or n < 4:
+        raise ValueError("find_sideband: holograms must be square with an even side")
+    out = np.empty((count, 5), np.float32)
+    _native.check(_native.lib().qh_hip_find_sidebands(_native.ptr(h), count, n,
+                                                      float(dc_exclusion), _native.ptr(out)))
+    return [Sideband(int(r[0]), int(r[1]), float(r[2]), float(r[3]), float(r[4])) for r in out]
+
+
+def estimate_illuminations(backgrounds, hologram_geometry: Geometry, dc_exclusion: float = 0.0):
+    """(carrier, illuminations): the carrier is the sideband of background 0 and illumination a
+    is (s_a - carrier) / (N pixel) in cycles per micrometre, as the reference's
+    examples/readme_figure.py:39-46 estimates the angles; on the device."""
+    h, _ = _stack(backgrounds, np.float32)
+    count, ny, n = h.shape
+    if ny != n or n % 2 or n < 4:
+        raise ValueError("estimate_illuminations: holograms must be square with an even side")
+    out = np.empty((count, 2), np.float32)
+    c = np.empty(5, np.float32)
+    _native.check(_native.lib().qh_hip_estimate_illuminations(
+        _native.ptr(h), count, n, float(hologram_geometry.pixel), float(dc_exclusion),
+        _native.ptr(out), _native.ptr(c)))
+    return Sideband(int(c[0]), int(c[1]), float(c[2]), float(c[3]), float(c[4])), out
 
 
 def _crop(n: int, center, radius: float, output_size: int) -> _native.CropPlan_t:
@@ -467,6 +502,16 @@ class Server:
                      nz: int = 0, iterations: int = 0, solver: str = "gp",
                      gp_transform: str = "r2c", tv_weight: float = 1e-2, tv_inner: int = 10,
                      mapping: str = "gather") -> int:
+        """Registers a geometry. With ``center`` or ``illuminations`` None they are estimated
+        from the backgrounds on the device (``estimate_illuminations``; the hologram pixel is
+        ``geometry.pixel * output_size / hologram_size``)."""
+        if center is None or illuminations is None:
+            m = output_size if output_size > 0 else hologram_size
+            hg = Geometry(geometry.wavelength, geometry.n_medium, geometry.na,
+                          geometry.pixel * m / hologram_size)
+            c, est = estimate_illuminations(backgrounds, hg)
+            center = (float(c.iy), float(c.ix)) if center is None else center
+            illuminations = est if illuminations is None else illuminations
         o, illum_c = _pipeline_options(count, hologram_size, center, illuminations, geometry,
                                        radius, output_size, nz, iterations, solver, gp_transform,
                                        tv_weight, tv_inner, mapping, 1)

@@ -95,6 +95,9 @@ _SIGNATURES = {
                                    POINTER(c_double), POINTER(OdtTimings_t)]),
     "qh_hip_map_kspace": (c_int, [c_void_p, c_int, c_int, c_void_p, POINTER(OdtGrid_t), c_void_p,
                                   c_void_p, POINTER(c_double)]),
+    "qh_hip_find_sidebands": (c_int, [c_void_p, c_int, c_int, c_float, c_void_p]),
+    "qh_hip_estimate_illuminations": (c_int, [c_void_p, c_int, c_int, c_float, c_float, c_void_p,
+                                              c_void_p]),
     "qh_hip_pipeline_create": (c_int, [POINTER(PipelineOptions_t), POINTER(c_void_p)]),
     "qh_hip_pipeline_destroy": (None, [c_void_p]),
     "qh_hip_pipeline_set_background": (c_int, [c_void_p, c_void_p]),

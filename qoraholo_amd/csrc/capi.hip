@@ -1,8 +1,11 @@
 // The C-ABI: argument checks with the reference's rules, status codes and per-thread error
 // messages, and the per-call entry points that copy host arrays in and out.
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
+#include <complex>
+#include <numbers>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -119,6 +122,47 @@ void check_plan(const qh_crop_plan& p, int n) {
           "extract_fields: output_size must be even and <= N");
   require(p.radius > 0 && p.radius <= static_cast<float>(p.out) / 2.0f,
           "extract_fields: crop radius must fit in the output window");
+}
+
+// src/cpu/field.cpp:24-33 (refine): Jacobsen's three-bin estimate with Candan's tan(pi/N)/(pi/N)
+// correction, clamped to half a bin; float arithmetic as the reference's host code.
+float refine_peak(std::complex<float> lo, std::complex<float> mid, std::complex<float> hi, int n) {
+  const std::complex<float> den = 2.0f * mid - lo - hi;
+  if (std::abs(den) == 0.0f) return 0.0f;
+  const float c = std::numbers::pi_v<float> / static_cast<float>(n);
+  const float d = ((lo - hi) / den).real() * std::tan(c) / c;
+  return std::clamp(d, -0.5f, 0.5f);
+}
+
+// Sideband (iy, ix, y, x, magnitude) of every hologram of a host stack.
+std::vector<std::array<float, 5>> device_sidebands(const float* holo, int count, int n,
+                                                   float dc_exclusion) {
+  require(holo != nullptr && count >= 1, "find_sideband: need at least one hologram");
+  require(n >= 4 && n % 2 == 0, "find_sideband: holograms must be square with an even side");
+  require_fft(n, "find_sideband");
+  if (dc_exclusion <= 0) dc_exclusion = static_cast<float>(n) / 8.0f;
+  require_device();
+  const hipStream_t s = call_stream();
+  const std::size_t plane = static_cast<std::size_t>(n) * n;
+  DeviceBuffer<float> dh(plane * count);
+  DeviceBuffer<float2> half(static_cast<std::size_t>(count) * n * (n / 2 + 1));
+  DeviceBuffer<SidebandPeak> peaks(count);
+  check(hipMemcpyAsync(dh.get(), holo, dh.bytes(), hipMemcpyHostToDevice, s), "upload");
+  sideband_peaks(dh.get(), count, n, dc_exclusion, half.get(), peaks.get(), s);
+  std::vector<SidebandPeak> h(count);
+  check(hipMemcpyAsync(h.data(), peaks.get(), peaks.bytes(), hipMemcpyDeviceToHost, s), "download");
+  check(hipStreamSynchronize(s), "find_sideband");
+  std::vector<std::array<float, 5>> out(count);
+  for (int a = 0; a < count; ++a) {
+    const SidebandPeak& p = h[a];
+    if (p.mag == 0.0f) throw DeviceError("find_sideband: no peak outside the DC region");
+    const auto cx = [](float2 v) { return std::complex<float>(v.x, v.y); };
+    out[a] = {static_cast<float>(p.iy), static_cast<float>(p.ix),
+              static_cast<float>(p.iy) + refine_peak(cx(p.s[1]), cx(p.s[0]), cx(p.s[2]), n),
+              static_cast<float>(p.ix) + refine_peak(cx(p.s[3]), cx(p.s[0]), cx(p.s[4]), n),
+              p.mag};
+  }
+  return out;
 }
 
 // src/api.cpp:141-157 (checked_bpm_tables): checks of bpm_forward and bpm_reconstruct.
@@ -584,6 +628,32 @@ int qh_hip_irfft3(const qh_c64* in, int nz, int ny, int nx, float scale, float* 
     irfft3(dh.get(), dx.get(), nz, ny, nx, scale, s);
     check(hipMemcpyAsync(x, dx.get(), dx.bytes(), hipMemcpyDeviceToHost, s), "download");
     check(hipStreamSynchronize(s), "irfft3");
+  });
+}
+
+int qh_hip_find_sidebands(const float* holograms, int count, int n, float dc_exclusion,
+                          float* out) {
+  return guarded([&] {
+    require(out != nullptr, "find_sideband: null output");
+    const auto r = device_sidebands(holograms, count, n, dc_exclusion);
+    for (int a = 0; a < count; ++a) std::memcpy(out + 5 * a, r[a].data(), 5 * sizeof(float));
+  });
+}
+
+int qh_hip_estimate_illuminations(const float* backgrounds, int count, int n,
+                                  float hologram_pixel, float dc_exclusion, qh_illumination* out,
+                                  float* carrier) {
+  return guarded([&] {
+    require(out != nullptr && carrier != nullptr, "estimate_illuminations: null output");
+    require(hologram_pixel > 0, "estimate_illuminations: pixel must be positive");
+    const auto r = device_sidebands(backgrounds, count, n, dc_exclusion);
+    // examples/readme_figure.py:39-44: df = 1 / (N pixel), f_a = (s_a - c) df, in double.
+    const double df = 1.0 / (static_cast<double>(n) * static_cast<double>(hologram_pixel));
+    for (int a = 0; a < count; ++a) {
+      out[a].fy = static_cast<float>((static_cast<double>(r[a][2]) - r[0][2]) * df);
+      out[a].fx = static_cast<float>((static_cast<double>(r[a][3]) - r[0][3]) * df);
+    }
+    std::memcpy(carrier, r[0].data(), 5 * sizeof(float));
   });
 }
 

@@ -63,7 +63,89 @@ __global__ void k_split_ratio(const float2* r, long total, float* psi, float* lo
   logamp[i] = logf(hypotf(v.x, v.y));
 }
 
+// Element (y, x) of the centred spectrum S = FFT((-1)^(x+y) h) read from the unmodulated half
+// spectrum: S[y][x] = H((y - c) mod n, x - c), columns left of the centre as conj(H(-f)).
+__device__ __forceinline__ float2 shifted(const float2* spec, int n, int y, int x) {
+  const int h = n / 2 + 1;
+  const int c = n / 2;
+  const int fy = y - c;
+  const int fx = x - c;
+  if (fx >= 0) return spec[static_cast<long>((fy + n) % n) * h + fx];
+  const float2 v = spec[static_cast<long>((n - fy) % n) * h - fx];
+  return {v.x, -v.y};
+}
+
+// Largest |S| over y in [1, n-1), x in [n/2, n-1), outside the DC disc and the half axis
+// dx = 0, dy <= 0 (src/cpu/field.cpp:35-49), one block per hologram. Each thread scans its
+// candidates in row-major order keeping the first maximum; the block keeps the larger
+// magnitude and, on a tie, the earlier position, so the result is the reference loop's.
+__global__ void __launch_bounds__(1024) k_sideband_peak(const float2* half, int n, float dc2,
+                                                        SidebandPeak* out) {
+  __shared__ float sm[1024];
+  __shared__ long si[1024];
+  const int img = static_cast<int>(blockIdx.x);
+  const int tid = static_cast<int>(threadIdx.x);
+  const float2* spec = half + static_cast<long>(img) * n * (n / 2 + 1);
+  const int c = n / 2;
+  const int w = n - 1 - c;
+  const long total = static_cast<long>(n - 2) * w;
+  float best = 0.0f;
+  long bi = -1;
+  for (long e = tid; e < total; e += blockDim.x) {
+    const int y = 1 + static_cast<int>(e / w);
+    const int x = c + static_cast<int>(e % w);
+    const int dy = y - c;
+    const int dx = x - c;
+    if (dx == 0 && dy <= 0) continue;
+    if (static_cast<float>(dy * dy + dx * dx) <= dc2) continue;
+    const float2 v = shifted(spec, n, y, x);
+    const float m = hypotf(v.x, v.y);
+    if (m > best) {
+      best = m;
+      bi = e;
+    }
+  }
+  sm[tid] = best;
+  si[tid] = bi;
+  __syncthreads();
+  for (int stride = static_cast<int>(blockDim.x) / 2; stride > 0; stride >>= 1) {
+    if (tid < stride) {
+      const float mo = sm[tid + stride];
+      const long io = si[tid + stride];
+      if (mo > sm[tid] || (mo == sm[tid] && io >= 0 && (si[tid] < 0 || io < si[tid]))) {
+        sm[tid] = mo;
+        si[tid] = io;
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    SidebandPeak r{};
+    r.mag = sm[0];
+    if (si[0] >= 0) {
+      const int y = 1 + static_cast<int>(si[0] / w);
+      const int x = c + static_cast<int>(si[0] % w);
+      r.iy = y - c;
+      r.ix = x - c;
+      r.s[0] = shifted(spec, n, y, x);
+      r.s[1] = shifted(spec, n, y - 1, x);
+      r.s[2] = shifted(spec, n, y + 1, x);
+      r.s[3] = shifted(spec, n, y, x - 1);
+      r.s[4] = shifted(spec, n, y, x + 1);
+    }
+    out[img] = r;
+  }
+}
+
 }  // namespace
+
+void sideband_peaks(const float* holo, int count, int n, float dc_exclusion, float2* half,
+                    SidebandPeak* peaks, hipStream_t s) {
+  hologram_spectra(holo, count, n, half, s);
+  hipLaunchKernelGGL(k_sideband_peak, dim3(count), dim3(1024), 0, s, half, n,
+                     dc_exclusion * dc_exclusion, peaks);
+  check_launch("sideband_peak");
+}
 
 void hologram_spectra(const float* holo, int count, int n, float2* half, hipStream_t s) {
   const int h = n / 2 + 1;

@@ -39,6 +39,18 @@ void ratio_phase(const float2* sample, const float2* background, long total, flo
                  float* logamp, hipStream_t s);
 // psi = arg r, logamp = log|r| of a given ratio.
 void split_ratio(const float2* ratio, long total, float* psi, float* logamp, hipStream_t s);
+// Sideband peak of each hologram of a stack (src/cpu/field.cpp:35-49 on the device): integer
+// offset from DC, magnitude, and the centred spectrum at the peak and its four neighbours
+// (centre, y-1, y+1, x-1, x+1) for the sub-bin refinement on the host. half: (count, n, n/2+1).
+struct SidebandPeak {
+  int iy;
+  int ix;
+  float mag;
+  int pad;
+  float2 s[5];
+};
+void sideband_peaks(const float* holo, int count, int n, float dc_exclusion, float2* half,
+                    SidebandPeak* peaks, hipStream_t s);
 // Plain 2D transform of (count, n, n) complex images in place.
 void fft2_images(float2* data, int count, int n, bool inverse, hipStream_t s);
 

@@ -94,13 +94,56 @@ def _ball(x):
     return np.where(small, 1.0 - x * x / 10.0, 3.0 * (np.sin(xs) - xs * np.cos(xs)) / (xs * xs * xs))
 
 
+def partial_waves(x: float, m: float) -> np.ndarray:
+    """Coefficients c_l of the scattered wave of a homogeneous sphere for the scalar Helmholtz
+    equation (phantom.cpp:41-80): outside u = sum i^l (2l+1) (j_l(kr) + c_l h_l(kr)) P_l,
+    inside b_l j_l(mkr), u and du/dr continuous at r = R; x = k R, m = n_sphere / n_medium.
+
+    j_l / j_{l-1} comes from the downward recurrence (started 60 orders above lmax, where the
+    upward one is unstable), y_l from the upward one; c_l = (m D j - j') / (h' - m D h) with
+    D = j_l'(mx) / j_l(mx)."""
+    lmax = int(x + 4.0 * np.cbrt(x) + 10.0)
+
+    def ratios(z):
+        rho = np.zeros(lmax + 1)
+        r = 0.0
+        for l in range(lmax + 60, 0, -1):
+            r = 1.0 / ((2.0 * l + 1.0) / z - r)
+            if l <= lmax:
+                rho[l] = r
+        return rho
+
+    rx = ratios(x)
+    rm = ratios(m * x)
+    c = np.zeros(lmax + 1, np.complex128)
+    j_prev, j = math.cos(x) / x, math.sin(x) / x
+    y_prev, y = math.sin(x) / x, -math.cos(x) / x
+    for l in range(lmax + 1):
+        if l > 0:
+            y_prev, y = y, (2.0 * l - 1.0) / x * y - y_prev
+            j_prev, j = j, rx[l] * j
+        h = complex(j, y)
+        h_prev = complex(j_prev, y_prev)
+        dj = j_prev - (l + 1.0) / x * j
+        dh = h_prev - (l + 1.0) / x * h
+        dlog = (1.0 / math.tan(m * x) if l == 0 else 1.0 / rm[l]) - (l + 1.0) / (m * x)
+        c[l] = (m * dlog * j - dj) / (dh - m * dlog * h)
+    return c
+
+
 def forward_fields(shapes, illuminations, geometry, n: int, model: str = "rytov") -> np.ndarray:
-    """Field ratios u / u_in at z = 0, (count, n, n) complex64 (phantom.cpp:102-191, born/rytov)."""
+    """Field ratios u / u_in at z = 0, (count, n, n) complex64 (phantom.cpp:102-191).
+
+    born: first Born; rytov: the Born phase as a Rytov phase; exact: one sphere solved by partial
+    waves (multiple scattering included), the forward-scattered plane waves at z = 0 through the
+    scattering amplitude f(theta) = -i / k sum (2l+1) c_l P_l(cos theta)."""
     if n < 4 or n % 2:
         raise ValueError("forward_fields: n must be even")
-    if model not in ("born", "rytov"):
-        raise ValueError("forward_fields: model must be 'born' or 'rytov'")
+    if model not in ("born", "rytov", "exact"):
+        raise ValueError("forward_fields: model must be 'born', 'rytov' or 'exact'")
     shapes = list(shapes)
+    if model == "exact" and (len(shapes) != 1 or not isinstance(shapes[0], Sphere)):
+        raise ValueError("forward_fields: the exact model takes exactly one sphere")
     wl = _f32(geometry.wavelength)
     nm = _f32(geometry.n_medium)
     k0 = 2.0 * math.pi / wl
@@ -108,6 +151,9 @@ def forward_fields(shapes, illuminations, geometry, n: int, model: str = "rytov"
     fna2 = _f32(geometry.na) ** 2 / (wl * wl)
     d = _f32(geometry.pixel)
     df = 1.0 / (n * d)
+    km = k0 * nm
+    waves = (partial_waves(km * _f32(shapes[0].radius), _f32(shapes[0].n) / nm)
+             if model == "exact" else None)
     steps = []
     for i, s in enumerate(shapes):
         parent = float(_index_at(shapes, i, nm, _f32(s.z), _f32(s.y), _f32(s.x)))
@@ -135,7 +181,19 @@ def forward_fields(shapes, illuminations, geometry, n: int, model: str = "rytov"
         nuz = np.sqrt(fm2 - rho2[mask])
         qz = nuz - fiz
         o = np.zeros(nuz.shape, np.complex128)
-        for s, st in zip(shapes, steps):
+        if model == "exact":
+            sp = shapes[0]
+            cos_t = (fy[mask] * fiy + fx[mask] * fix + nuz * fiz) / fm2
+            total = np.zeros(nuz.shape, np.complex128)
+            p_prev = np.zeros(nuz.shape)
+            p = np.ones(nuz.shape)
+            for l, cl in enumerate(waves):
+                w = 2.0 * l + 1.0
+                total = total + w * cl * p
+                p_prev, p = p, (w * cos_t * p - l * p_prev) / (l + 1.0)
+            ph = -2.0 * math.pi * (qz * _f32(sp.z) + myv * _f32(sp.y) + mxv * _f32(sp.x))
+            o = 4.0 * math.pi * (-1j / km) * total * (np.cos(ph) + 1j * np.sin(ph))
+        for s, st in zip(shapes if model != "exact" else [], steps):
             ph = -2.0 * math.pi * (qz * _f32(s.z) + myv * _f32(s.y) + mxv * _f32(s.x))
             if isinstance(s, Sphere):
                 q = 2.0 * math.pi * np.sqrt(qz * qz + myv * myv + mxv * mxv)

@@ -208,3 +208,22 @@ def test_full_size_invariants(device, qh):
     assert torch.equal(stored.vjp(x, r1), g1)
     del stored
     torch.cuda.synchronize()
+
+
+def test_inversion_of_exact_bead_on_device(device, qh, oracle):
+    """The reference's exact-bead inversion (tests/test_bpm.cpp:131-171) with the device path
+    end to end (Rytov, GP 20 start, 20 BPM iterations): the reference's measured values and the
+    oracle's volume."""
+    from test_oracle_bpm import exact_bead_case
+
+    g, il, data, truth, initial_o, inner = exact_bead_case(oracle, qh)
+    initial = qh.reconstruct(qh.rytov(data), il, g, iterations=20)
+    nm = np.float32(GEOM[1])
+    assert rel_l2(initial - nm, initial_o - nm) < 5e-5
+    ri, loss = qh.bpm_reconstruct(data, il, g, initial, return_loss=True)
+    want, wl = oracle.bpm_reconstruct(data, il, GEOM, initial_o)
+    assert rel_l2(ri - nm, want - nm) < 1e-4
+    assert np.allclose(loss, wl, rtol=1e-4)
+    rmse = float(np.sqrt(np.mean((ri.astype(np.float64) - truth) ** 2)))
+    assert rmse == pytest.approx(3.21e-3, abs=5e-5)
+    assert float(ri[inner].mean()) == pytest.approx(1.4019, abs=1e-4)

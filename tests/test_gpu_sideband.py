@@ -1,0 +1,69 @@
+"""Device-side sideband search and illumination estimation against the oracle's find_sideband
+(src/cpu/field.cpp:35-60) and the reference's own known answer (tests/test_field.cpp:39-50),
+and the served loop's cold start from backgrounds alone."""
+
+import numpy as np
+import pytest
+
+from test_gpu_parity import PipelineData
+
+pytestmark = pytest.mark.gpu
+
+
+def test_carrier_sideband_known_answer(device, qh, oracle):
+    """tests/test_field.cpp:39-50: 128^2 background at 0.055 um, carrier 5 cycles/um: peak
+    (35, 35), refined to 35.2 px within 0.05; the same values as the oracle."""
+    g = qh.Geometry(0.532, 1.337, 1.2, 0.055)
+    bg = qh.synthesize_holograms(np.ones((1, 128, 128), np.complex64), [[0, 0]], g,
+                                 carrier=(5.0, 5.0))
+    (s,) = qh.find_sidebands(bg)
+    assert (s.iy, s.ix) == (35, 35)
+    expected = 5.0 * 128 * 0.055
+    assert abs(s.y - expected) < 0.05 and abs(s.x - expected) < 0.05
+    iy, ix, y, x, mag = oracle.find_sideband(bg)
+    assert abs(s.y - y) < 1e-4 and abs(s.x - x) < 1e-4
+    assert s.magnitude == pytest.approx(mag, rel=1e-5)
+
+
+@pytest.mark.parametrize("holo,angles", [(128, 9), (256, 25), (512, 49)])
+def test_every_background_matches_oracle(device, qh, oracle, holo, angles):
+    d = PipelineData(qh, holo=holo, angles=angles)
+    got = qh.find_sidebands(d.bg)
+    for a, s in enumerate(got):
+        iy, ix, y, x, mag = oracle.find_sideband(d.bg[a])
+        assert (s.iy, s.ix) == (iy, ix), a
+        assert abs(s.y - y) < 2e-4 and abs(s.x - x) < 2e-4, a
+        assert s.magnitude == pytest.approx(mag, rel=1e-4)
+    # A second DC exclusion radius moves nothing for peaks well outside it.
+    wide = qh.find_sidebands(d.bg, dc_exclusion=float(holo) / 6.0)
+    assert [(s.iy, s.ix) for s in wide] == [(s.iy, s.ix) for s in got]
+
+
+def test_illuminations_from_backgrounds(device, qh, oracle):
+    """examples/readme_figure.py:39-46 on the device: the carrier and every illumination
+    frequency as the host estimate gives them, and within a fraction of a bin of the truth."""
+    d = PipelineData(qh, holo=256, angles=25)
+    carrier, est = qh.estimate_illuminations(d.bg, d.holo_geom)
+    df = 1.0 / (256 * 0.1)
+    c = oracle.find_sideband(d.bg[0])
+    host = np.array([[(oracle.find_sideband(b)[2] - c[2]) * df,
+                      (oracle.find_sideband(b)[3] - c[3]) * df] for b in d.bg], np.float32)
+    assert (carrier.iy, carrier.ix) == (c[0], c[1])
+    assert np.max(np.abs(est - host)) < 5e-4 * df
+    assert np.max(np.abs(est - d.illum)) < 0.5 * df
+
+
+def test_server_cold_start_from_backgrounds(device, qh):
+    """A geometry registered with backgrounds only (carrier and illuminations estimated on the
+    device) reconstructs as a Pipeline built from the same estimates."""
+    d = PipelineData(qh, holo=128, angles=9)
+    carrier, est = qh.estimate_illuminations(d.bg, d.holo_geom)
+    kw = dict(d.kw)
+    kw.pop("center")
+    server = qh.Server(max_slots=2)
+    gid = server.add_geometry(d.count, d.size, None, None, d.geom, d.bg, **kw)
+    ri, _, _ = server.wait(server.submit(gid, d.holo))
+    p = qh.Pipeline(d.count, d.size, center=(float(carrier.iy), float(carrier.ix)),
+                    illuminations=est, geometry=d.geom, **kw)
+    p.set_background(d.bg)
+    assert np.array_equal(ri, p.run(d.holo))

@@ -202,3 +202,51 @@ def test_reconstruction_lowers_the_loss_and_recovers_a_bead(oracle, qh):
         assert loss.shape == (8,) and loss[-1] < 0.2 * loss[0], (batch, tv, loss)
         assert np.sqrt(np.mean((ri - truth) ** 2)) < 0.5 * np.sqrt(np.mean((start - truth) ** 2))
         assert ri.min() >= np.float32(GEOM[1])
+
+
+def exact_bead_case(oracle, qh):
+    """tests/test_bpm.cpp:131-171: a 3 um, n = 1.40 bead, 64^3 at 0.2 um, 49 illuminations at
+    NA 0.9, data from the exact partial-wave model, Rytov + GP (20) start."""
+    from qoraholo_amd import phantom as ph
+
+    g = qh.Geometry(*GEOM)
+    bead = [ph.Sphere(0.0, 0.0, 0.0, 3.0, 1.40)]
+    il = oracle.circular_scan(49, 0.9, GEOM[0])
+    data = ph.forward_fields(bead, il, g, 64, model="exact")
+    truth = ph.render_phantom(bead, g, 64, 64)
+    initial, _ = oracle.reconstruct(oracle.rytov(data), il, GEOM, iterations=20)
+    z, y, x = np.indices(truth.shape) - 32
+    inner = (z * z + y * y + x * x) * GEOM[3] ** 2 < 0.36 * 3.0 ** 2
+    return g, il, data, truth, initial, inner
+
+
+def test_exact_model_matches_born_for_weak_sphere(qh):
+    """tests/test_bpm.cpp:29-36: the partial-wave solution of a weak sphere is the first Born
+    field to 2%."""
+    from qoraholo_amd import phantom as ph
+
+    g = qh.Geometry(*GEOM)
+    s = [ph.Sphere(0.4, -0.6, 0.2, 3.0, 1.3375)]
+    il = qh.circular_scan(5, 0.9, g)
+    exact = ph.forward_fields(s, il, g, 64, model="exact")
+    born = ph.forward_fields(s, il, g, 64, model="born")
+    e = np.sqrt(np.sum(np.abs(exact.astype(np.complex128) - born) ** 2) /
+                np.sum(np.abs(born.astype(np.complex128) - 1) ** 2))
+    assert e < 0.02
+
+
+def test_inversion_of_exact_bead_known_answer(oracle, qh):
+    """The reference's measured values (tests/test_bpm.cpp:166-167): RMSE 6.04e-3 (Rytov + GP)
+    -> 3.21e-3, inner mean 1.4030 -> 1.4019, 20 losses, the last below a tenth of the first."""
+    _, il, data, truth, initial, inner = exact_bead_case(oracle, qh)
+    ri, loss = oracle.bpm_reconstruct(data, il, GEOM, initial)
+
+    def rmse(v):
+        return float(np.sqrt(np.mean((v.astype(np.float64) - truth) ** 2)))
+
+    assert rmse(initial) == pytest.approx(6.04e-3, abs=5e-5)
+    assert rmse(ri) == pytest.approx(3.21e-3, abs=5e-5)
+    assert rmse(ri) < 0.6 * rmse(initial)
+    assert float(initial[inner].mean()) == pytest.approx(1.4030, abs=1e-4)
+    assert float(ri[inner].mean()) == pytest.approx(1.4019, abs=1e-4)
+    assert loss.shape == (20,) and loss[-1] < 0.1 * loss[0]
