@@ -10,6 +10,25 @@ from test_gpu_parity import PipelineData
 pytestmark = pytest.mark.gpu
 
 
+def refined64(h, iy, ix):
+    """Sub-bin position of the peak (iy, ix) from the double-precision spectrum of the
+    modulated hologram (src/cpu/field.cpp:24-33 in FP64): the value both float paths round."""
+    n = h.shape[-1]
+    k = np.arange(n)
+    sign = np.where((k[:, None] + k[None, :]) % 2 == 0, 1.0, -1.0)
+    s = np.fft.fft2(h.astype(np.float64) * sign)
+    c = n // 2
+    y, x = iy + c, ix + c
+
+    def refine(lo, mid, hi):
+        den = 2.0 * mid - lo - hi
+        cc = np.pi / n
+        return float(np.clip(((lo - hi) / den).real * np.tan(cc) / cc, -0.5, 0.5))
+
+    return (iy + refine(s[y - 1, x], s[y, x], s[y + 1, x]),
+            ix + refine(s[y, x - 1], s[y, x], s[y, x + 1]))
+
+
 def test_carrier_sideband_known_answer(device, qh, oracle):
     """tests/test_field.cpp:39-50: 128^2 background at 0.055 um, carrier 5 cycles/um: peak
     (35, 35), refined to 35.2 px within 0.05; the same values as the oracle."""
@@ -27,12 +46,18 @@ def test_carrier_sideband_known_answer(device, qh, oracle):
 
 @pytest.mark.parametrize("holo,angles", [(128, 9), (256, 25), (512, 49)])
 def test_every_background_matches_oracle(device, qh, oracle, holo, angles):
+    """Integer peaks equal the oracle's; the refined positions are as close to the FP64 value as
+    the float oracle's are (within 2x, plus 1e-4 bin). Peaks next to the DC disc make the
+    three-bin estimate ill-conditioned (a denominator of a few units against peaks of ~700),
+    where both float paths move by up to 1e-3 bin."""
     d = PipelineData(qh, holo=holo, angles=angles)
     got = qh.find_sidebands(d.bg)
     for a, s in enumerate(got):
         iy, ix, y, x, mag = oracle.find_sideband(d.bg[a])
         assert (s.iy, s.ix) == (iy, ix), a
-        assert abs(s.y - y) < 2e-4 and abs(s.x - x) < 2e-4, a
+        y64, x64 = refined64(d.bg[a], iy, ix)
+        assert abs(s.y - y64) <= 2 * abs(y - y64) + 1e-4, (a, s.y, y, y64)
+        assert abs(s.x - x64) <= 2 * abs(x - x64) + 1e-4, (a, s.x, x, x64)
         assert s.magnitude == pytest.approx(mag, rel=1e-4)
     # A second DC exclusion radius moves nothing for peaks well outside it.
     wide = qh.find_sidebands(d.bg, dc_exclusion=float(holo) / 6.0)
@@ -46,10 +71,12 @@ def test_illuminations_from_backgrounds(device, qh, oracle):
     carrier, est = qh.estimate_illuminations(d.bg, d.holo_geom)
     df = 1.0 / (256 * 0.1)
     c = oracle.find_sideband(d.bg[0])
-    host = np.array([[(oracle.find_sideband(b)[2] - c[2]) * df,
-                      (oracle.find_sideband(b)[3] - c[3]) * df] for b in d.bg], np.float32)
     assert (carrier.iy, carrier.ix) == (c[0], c[1])
-    assert np.max(np.abs(est - host)) < 5e-4 * df
+    pos = [oracle.find_sideband(b) for b in d.bg]
+    exact = [refined64(b, p[0], p[1]) for b, p in zip(d.bg, pos)]
+    host = np.array([[(p[2] - c[2]) * df, (p[3] - c[3]) * df] for p in pos])
+    truth = np.array([[(e[0] - exact[0][0]) * df, (e[1] - exact[0][1]) * df] for e in exact])
+    assert np.all(np.abs(est - truth) <= 2 * np.abs(host - truth) + 2e-4 * df)
     assert np.max(np.abs(est - d.illum)) < 0.5 * df
 
 
