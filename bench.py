@@ -41,10 +41,10 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def fgp_traffic():
-    """HBM bytes per FGP launch measured by rocprofv3 (FETCH_SIZE + WRITE_SIZE, scaled by the
-    copy-kernel calibration), from the committed summary when it exists."""
-    path = ROOT / "profiles" / "fgp_step_256_hbm_traffic.txt"
+def fgp_traffic(kernel: str):
+    """HBM bytes per launch of the FGP kernel measured by rocprofv3 (FETCH_SIZE + WRITE_SIZE,
+    scaled by the copy-kernel calibration), from the committed summary when it exists."""
+    path = ROOT / "profiles" / f"{kernel}_256_hbm_traffic.txt"
     if not path.exists():
         return None
     m = re.search(r"traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", path.read_text())
@@ -77,19 +77,21 @@ def cpu_baseline(w):
 
 
 def fgp_traffic_run(qh, torch) -> int:
-    """5 FGP steps on a 256^3 volume (plus the prox's fill and primal kernels) and one copy of a
-    256 MiB buffer (reads and writes exactly 256 MiB) as the counter calibration."""
+    """One TV prox of 10 FGP steps on a 256^3 volume (five two-step passes, plus the prox's fill
+    and primal kernels) and one copy of a 256 MiB buffer (reads and writes exactly 256 MiB) as
+    the counter calibration."""
     from qoraholo_amd import _native
 
     rng = np.random.default_rng(0)
     b = (rng.standard_normal((N, N, N)) * 0.01).astype(np.float32)
     x = np.empty_like(b)
-    _native.check(_native.lib().qh_hip_tv_prox(_native.ptr(b), N, N, N, 0.5, 5, _native.ptr(x)))
+    _native.check(_native.lib().qh_hip_tv_prox(_native.ptr(b), N, N, N, 0.5, 10, 1,
+                                                _native.ptr(x)))
     src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda:0")
     dst = torch.empty_like(src)
     dst.copy_(src)
     torch.cuda.synchronize()
-    log("fgp traffic run: 5 FGP steps at 256^3 and a 268435456-byte copy")
+    log("fgp traffic run: 10 FGP steps in 5 passes at 256^3 and a 268435456-byte copy")
     return 0
 
 
@@ -189,6 +191,84 @@ def bpm_leg(qh, torch, cpu=True, repeats=5):
                                          f"{secs:.2f} s, scaled to {a}"}
         out["parity"] = {"scattered_rel_l2_vs_oracle": err, "illuminations": sample}
         log(f"bpm: oracle forward on {sample} illuminations {secs:.2f} s; parity {err:.2e}")
+    return out
+
+
+def served_mixed(qh, rates=(0.25, 0.5, 1.0, 2.0), per_rate=16, seed=0):
+    """Config 3: a served stream of mixed shapes and solvers at Poisson rates set against the
+    plain path's measured capacity. Shapes uniform over 25 x 256^2 -> 128^3, 49 x 512^2 -> 256^3
+    and 81 x 768^2 -> 384^3 (random cells of 3-8 nested or disjoint spheres), solvers uniform
+    over GP 20 and FISTA-TV 20 x 10. Capacity = 1 / mean service time of the plain per-call
+    chain over the six kinds; also reported: serial Pipeline.run per kind."""
+    from qoraholo_amd import workloads
+
+    kinds = []
+    t0 = time.perf_counter()
+    for i, (angles, n) in enumerate(((25, 128), (49, 256), (81, 384))):
+        w = workloads.mixed_cell(n, angles, seed=seed + i)
+        for name, kw in (("gp", dict(iterations=ITERATIONS, solver="gp")),
+                         ("tv", dict(iterations=ITERATIONS, solver="tv", tv_weight=TV_WEIGHT,
+                                     tv_inner=INNER))):
+            kinds.append((f"{n}^3 {name}", w, kw))
+    log(f"mixed stream: workloads in {time.perf_counter() - t0:.1f} s")
+
+    def plain_once(w, kw):
+        crop = dict(center=w.center, radius=w.radius, output_size=w.field_size)
+        ratio = qh.field_ratio(qh.extract_fields(w.holograms, **crop),
+                               qh.extract_fields(w.backgrounds, **crop))
+        return qh.reconstruct(qh.rytov(ratio), w.illuminations, w.geometry, **kw)
+
+    plain_ms, pipe_ms = {}, {}
+    for name, w, kw in kinds:
+        plain_once(w, kw)
+        s0 = time.perf_counter()
+        plain_once(w, kw)
+        plain_ms[name] = 1000.0 * (time.perf_counter() - s0)
+        p = qh.Pipeline(w.holograms.shape[0], w.holograms.shape[1], w.center, w.illuminations,
+                        w.geometry, radius=w.radius, output_size=w.field_size, **kw)
+        p.set_background(w.backgrounds)
+        p.run(w.holograms)
+        s0 = time.perf_counter()
+        p.run(w.holograms)
+        pipe_ms[name] = 1000.0 * (time.perf_counter() - s0)
+        del p
+    capacity = 1000.0 / float(np.mean(list(plain_ms.values())))
+    log(f"mixed stream: plain ms {plain_ms}, pipeline ms {pipe_ms}, capacity {capacity:.2f}/s")
+
+    server = qh.Server(max_slots=4)
+    gids = [server.add_geometry(w.holograms.shape[0], w.holograms.shape[1], w.center,
+                                w.illuminations, w.geometry, w.backgrounds, radius=w.radius,
+                                output_size=w.field_size, **kw) for _, w, kw in kinds]
+    for k, (_, w, _) in enumerate(kinds):
+        server.wait(server.submit(gids[k], w.holograms))
+    rng = np.random.default_rng(seed)
+    out = {"kinds": [k[0] for k in kinds], "plain_ms": {k: round(v, 2) for k, v in plain_ms.items()},
+           "pipeline_ms": {k: round(v, 2) for k, v in pipe_ms.items()},
+           "plain_capacity_per_s": round(capacity, 3), "rates": []}
+    for factor in rates:
+        rate = factor * capacity
+        server.reset_stats()
+        order = rng.integers(0, len(kinds), per_rate)
+        gaps = rng.exponential(1.0 / rate, per_rate)
+        ids, peak_queue = [], 0
+        tick = time.perf_counter()
+        for k, gap in zip(order, gaps):
+            tick += gap
+            delay = tick - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            ids.append(server.submit(gids[int(k)], kinds[int(k)][1].holograms))
+            peak_queue = max(peak_queue, server.stats()["queued"])
+        for rid in ids:
+            server.wait(rid)
+        st = server.stats()
+        row = {"rate_x_capacity": factor, "rate_per_s": round(rate, 3),
+               "recon_s": round(st["recon_per_s"], 3), "p50_ms": round(st["p50_ms"], 2),
+               "p99_ms": round(st["p99_ms"], 2), "requests": per_rate, "admitted": st["completed"],
+               "peak_queued": peak_queue, "slots": st["slots"],
+               "device_gb": round(st["device_bytes"] / 1e9, 2)}
+        out["rates"].append(row)
+        log(f"mixed stream at {factor}x: {row}")
     return out
 
 
@@ -297,6 +377,9 @@ def main() -> int:
     ap.add_argument("--no-bpm", action="store_true", help="skip the beam propagation leg")
     ap.add_argument("--bpm-only", action="store_true",
                     help="run only the beam propagation leg and print its JSON object")
+    ap.add_argument("--served-mixed", action="store_true",
+                    help="run only the mixed served stream (128^3/256^3/384^3, GP and TV, "
+                         "Poisson arrivals at 0.25x-2x the plain capacity) and print its JSON")
     ap.add_argument("--fgp-traffic", action="store_true",
                     help="run only 5 FGP steps at 256^3 and a 256 MiB device copy, for the "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE runs that calibrate HBM traffic")
@@ -317,6 +400,9 @@ def main() -> int:
 
     if not qh.hip_available():
         raise SystemExit("no HIP device or libqoraholo_hip.so missing")
+    if args.served_mixed:
+        print(json.dumps(served_mixed(qh)), flush=True)
+        return 0
     if args.bpm_only:
         print(json.dumps(bpm_leg(qh, torch, cpu=not args.no_cpu_baseline, repeats=args.steps)),
               flush=True)
@@ -361,19 +447,29 @@ def main() -> int:
     p50, p99 = np.percentile(lat, [50, 99])
 
     vol = N * N * N
-    fgp_bytes = 13 * vol * 4  # read b, r(3), p(3); write p(3), r(3): fp32 volumes
-    fgp_avg_ms = t["fgp_ms"] / max(1, t["fgp_steps"])
+    launches = max(1, t["fgp_launches"])
+    steps_per_launch = t["fgp_steps"] / launches
+    kernel = "fgp_pair" if steps_per_launch == 2 else "fgp_step"
+    # The reference's per-step model: 13 fp32 volumes (read b, r(3), p(3); write p(3), r(3)),
+    # times the steps one launch performs.
+    fgp_bytes = int(13 * vol * 4 * steps_per_launch)
+    # What the launch must move at least: b and two dual iterates in, its new iterates out.
+    min_bytes = int((7 + 3 * steps_per_launch) * vol * 4)
+    fgp_avg_ms = t["fgp_ms"] / launches
     achieved = fgp_bytes / (fgp_avg_ms * 1e-3) / 1e9
     roofline = {
         "bound": "hbm",
-        "kernel": "k_fgp_step",
+        "kernel": f"k_{kernel}",
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": fgp_traffic(),
+        "traffic": fgp_traffic(kernel),
         "launch_ms": round(fgp_avg_ms, 5),
+        "steps_per_launch": steps_per_launch,
         "algorithmic_bytes": fgp_bytes,
+        "min_bytes": min_bytes,
+        "min_bytes_gbs": round(min_bytes / (fgp_avg_ms * 1e-3) / 1e9, 1),
     }
     stages = {k: round(v, 3) for k, v in t.items() if k not in ("fgp_steps",)}
 

@@ -82,7 +82,8 @@ typedef struct {
 } qh_odt_options;
 
 /* src/backends.hpp:31-42 (OdtTimings), device event intervals in milliseconds, plus the summed
- * duration of the FGP dual-step kernels and their count (FISTA-TV only). */
+ * duration of the FGP dual-step kernels, the steps and the kernel passes they took (two steps
+ * per pass where the row width allows; FISTA-TV only). */
 typedef struct {
   size_t bytes;
   double setup;
@@ -96,6 +97,7 @@ typedef struct {
   double download;
   double fgp_ms;
   int fgp_steps;
+  int fgp_launches;
 } qh_odt_timings;
 
 /* include/qoraholo/pipeline.hpp:31-43 (PipelineTimings), plus the FGP kernel totals. */
@@ -113,6 +115,7 @@ typedef struct {
   double total;
   double fgp_ms;
   int fgp_steps;
+  int fgp_launches;
 } qh_pipeline_timings;
 
 /* Message of the last failed call on this thread ("" when none). */
@@ -327,8 +330,10 @@ int qh_hip_fft_c2c(qh_c64* data, int d0, int d1, int d2, unsigned axes, int forw
 int qh_hip_rfft3(const float* x, int nz, int ny, int nx, qh_c64* out);
 int qh_hip_irfft3(const qh_c64* in, int nz, int ny, int nx, float scale, float* x);
 /* One TV prox call from a zero dual: argmin 0.5 |x - b|^2 + lambda TV(x), x >= 0
- * (src/cpu/tv.cpp:7-44). */
-int qh_hip_tv_prox(const float* b, int nz, int ny, int nx, float lambda, int inner, float* x);
+ * (src/cpu/tv.cpp:7-44). fused: 1 runs the FGP steps two per pass where the row width allows
+ * (what the solvers do), 0 one per pass; the results are the same bits. */
+int qh_hip_tv_prox(const float* b, int nz, int ny, int nx, float lambda, int inner, int fused,
+                   float* x);
 
 #ifdef __cplusplus
 }

@@ -48,13 +48,13 @@ class OdtOptions_t(ctypes.Structure):
 class OdtTimings_t(ctypes.Structure):
     _fields_ = [("bytes", c_size_t)] + [(k, c_double) for k in (
         "setup", "upload", "fft2", "clear", "mapping", "normalize", "solve", "finish", "download",
-        "fgp_ms")] + [("fgp_steps", c_int)]
+        "fgp_ms")] + [("fgp_steps", c_int), ("fgp_launches", c_int)]
 
 
 class PipelineTimings_t(ctypes.Structure):
     _fields_ = [(k, c_double) for k in (
         "upload", "fields", "rytov", "fft2", "clear", "mapping", "normalize", "solve", "finish",
-        "download", "total", "fgp_ms")] + [("fgp_steps", c_int)]
+        "download", "total", "fgp_ms")] + [("fgp_steps", c_int), ("fgp_launches", c_int)]
 
 
 class ServerStats_t(ctypes.Structure):
@@ -141,7 +141,7 @@ _SIGNATURES = {
     "qh_hip_fft_c2c": (c_int, [c_void_p, c_int, c_int, c_int, c_uint, c_int, c_float]),
     "qh_hip_rfft3": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p]),
     "qh_hip_irfft3": (c_int, [c_void_p, c_int, c_int, c_int, c_float, c_void_p]),
-    "qh_hip_tv_prox": (c_int, [c_void_p, c_int, c_int, c_int, c_float, c_int, c_void_p]),
+    "qh_hip_tv_prox": (c_int, [c_void_p, c_int, c_int, c_int, c_float, c_int, c_int, c_void_p]),
 }
 
 _lib = None

@@ -397,6 +397,7 @@ int qh_hip_reconstruct(const qh_c64* fields, int count, int n, const qh_illumina
       timings->download = clock.ms(kFinish, downloaded);
       timings->fgp_ms = solver.collect_fgp_ms();
       timings->fgp_steps = solver.fgp_steps();
+      timings->fgp_launches = solver.fgp_launches();
     }
   });
 }
@@ -814,7 +815,8 @@ int qh_hip_bpm_gradient_dev(qh_bpm_plan* plan, const float* x, const qh_c64* fie
   });
 }
 
-int qh_hip_tv_prox(const float* b, int nz, int ny, int nx, float lambda, int inner, float* x) {
+int qh_hip_tv_prox(const float* b, int nz, int ny, int nx, float lambda, int inner, int fused,
+                   float* x) {
   return guarded([&] {
     require(b && x && nz >= 1 && ny >= 1 && nx >= 1 && lambda > 0 && inner >= 0,
             "tv_prox: bad arguments");
@@ -824,6 +826,7 @@ int qh_hip_tv_prox(const float* b, int nz, int ny, int nx, float lambda, int inn
     DeviceBuffer<float> db(vol);
     DeviceBuffer<float> dx(vol);
     TvState state(nz, ny, nx);
+    state.set_fused(fused != 0);
     state.reset(s);
     check(hipMemcpyAsync(db.get(), b, db.bytes(), hipMemcpyHostToDevice, s), "upload");
     state.prox(db.get(), dx.get(), lambda, inner, s);

@@ -226,6 +226,7 @@ void OdtSolver::run(float2* fields, hipStream_t s, StageClock* clock, int first)
   const int h = n / 2 + 1;
   const float inv = 1.0f / static_cast<float>(vol_);
   fgp_steps_ = 0;
+  fgp_launches_ = 0;
   fgp_ms_ = 0;
 
   if (real_) {
@@ -284,7 +285,10 @@ void OdtSolver::run(float2* fields, hipStream_t s, StageClock* clock, int first)
     }
     mark(kSolve);
     mark(kFinish);
-    if (time_fgp_) fgp_steps_ = c_.iterations * c_.tv_inner;
+    if (time_fgp_) {
+      fgp_steps_ = c_.iterations * c_.tv_inner;
+      fgp_launches_ = c_.iterations * tv_state_->launches();
+    }
     return;
   }
 

@@ -162,6 +162,7 @@ void Pipeline::collect(bool downloaded) {
   timings_.download = downloaded ? clock_.ms(o + kFinish, kDownloaded) : 0.0;
   timings_.fgp_ms = solver_.collect_fgp_ms();
   timings_.fgp_steps = solver_.fgp_steps();
+  timings_.fgp_launches = solver_.fgp_launches();
 }
 
 double Pipeline::filled() const {

@@ -123,6 +123,7 @@ class OdtSolver {
   // Reads the FGP events of the last run; call after the stream has finished.
   double collect_fgp_ms();
   int fgp_steps() const { return fgp_steps_; }
+  int fgp_launches() const { return fgp_launches_; }
   void time_fgp(bool on) { time_fgp_ = on; }
 
  private:
@@ -150,6 +151,7 @@ class OdtSolver {
   bool time_fgp_ = false;
   double fgp_ms_ = 0;
   int fgp_steps_ = 0;
+  int fgp_launches_ = 0;
 };
 
 }  // namespace qh

@@ -259,6 +259,288 @@ __global__ void __launch_bounds__(64) k_fgp_step(TvArgs a, int chunks) {
   }
 }
 
+// ---- two FGP steps per pass ----
+//
+// A block of NW waves owns NW - 4 output rows (NW - 5 when the pass also writes the primal);
+// wave w holds row y0 - 2 + w (y0 - 3 + w) across its 64 lanes
+// (VX floats each) and the block marches z through a chunk of planes. Per plane t the waves
+// load b, p_k, p_{k-1} of plane t + 2, form r_k and z_k there, the dual p_{k+1} of plane t + 1
+// and its r_{k+1}, z_{k+1} of plane t + 1, and p_{k+2} of plane t. y neighbours come through
+// LDS (one row buffer per quantity, read back before the owner overwrites it), x neighbours by
+// shuffles, z neighbours from the registers of the previous plane. The two halo rows on each
+// side make step one available where step two needs it; their results are not stored. Every
+// value is formed with the float operations of k_fgp_step, so a pass equals two single steps
+// bit for bit, while HBM sees 7 volumes read and 6 written per two steps instead of 20.
+constexpr int kPairChunk = 32;
+
+struct PairArgs {
+  const float* b;
+  const float* p_k;    // p_k (r_k is formed from it)
+  const float* p_km1;  // p_{k-1} (unused on the first step of a prox)
+  float* p_k1;         // out: p_{k+1}
+  float* p_k2;         // out: p_{k+2}
+  long vol;
+  int nz, ny, nx;
+  float lambda;
+  float step;
+  float m1;  // momentum of r_k = p_k + m1 (p_k - p_{k-1})
+  int first;
+  float m2;  // momentum of r_{k+1} = p_{k+1} + m2 (p_{k+1} - p_k)
+  // Last pass of a prox: p_k1 may be null (p_{k+1} is not needed afterwards) and out, when
+  // given, receives max(b - lambda L(p_{k+2}), 0) (or its refractive index with to_index).
+  float* out;
+  int to_index;
+  float nm2;
+  float inv_k02;
+};
+
+// z = max(b - lambda L(r), 0) on one row of plane zp (the expression order of primal_plane).
+template <int VX>
+__device__ __forceinline__ void pair_primal(const PairArgs& a, int zp, int y, int xb,
+                                            const float (&R)[3][VX], const float (&rz_below)[VX],
+                                            const float (&ry_up)[VX], const float (&B)[VX],
+                                            float (&Z)[VX]) {
+  const float left = __shfl_up(R[2][VX - 1], 1);
+#pragma unroll
+  for (int v = 0; v < VX; ++v) {
+    const int x = xb + v;
+    float s = 0.0f;
+    if (zp < a.nz - 1) s += R[0][v];
+    if (zp > 0) s -= rz_below[v];
+    if (y < a.ny - 1) s += R[1][v];
+    if (y > 0) s -= ry_up[v];
+    if (x < a.nx - 1) s += R[2][v];
+    if (x > 0) s -= (v == 0 ? left : R[2][v - 1]);
+    Z[v] = (y >= 0 && y < a.ny && x < a.nx && zp >= 0 && zp < a.nz) ? tv_primal(B[v], a.lambda, s)
+                                                                   : 0.0f;
+  }
+}
+
+// p' = P(r + step g) on one row of plane zp, g from z of this plane (zc), the plane above (zn)
+// and the row below (zdn): the dual step of k_fgp_step.
+template <int VX>
+__device__ __forceinline__ void pair_dual(const PairArgs& a, int zp, int y, int xb,
+                                          const float (&zc)[VX], const float (&zn)[VX],
+                                          const float (&zdn)[VX], const float (&R)[3][VX],
+                                          float (&P)[3][VX]) {
+  const float right = __shfl_down(zc[0], 1);
+#pragma unroll
+  for (int v = 0; v < VX; ++v) {
+    const int x = xb + v;
+    const float c = zc[v];
+    float g[3];
+    g[0] = zp < a.nz - 1 ? c - zn[v] : 0.0f;
+    g[1] = y < a.ny - 1 ? c - zdn[v] : 0.0f;
+    g[2] = x < a.nx - 1 ? c - (v == VX - 1 ? right : zc[v + 1]) : 0.0f;
+    float p3[3] = {0.0f, 0.0f, 0.0f};
+    float r3[3] = {R[0][v], R[1][v], R[2][v]};
+    tv_dual_step(g, a.step, 0.0f, p3, r3);
+#pragma unroll
+    for (int c3 = 0; c3 < 3; ++c3) P[c3][v] = p3[c3];
+  }
+}
+
+template <int VX>
+__device__ __forceinline__ void lds_put(float* buf, int row, int nx, int xb, const float (&v)[VX]) {
+#pragma unroll
+  for (int i = 0; i < VX; ++i) {
+    if (xb + i < nx) buf[row * nx + xb + i] = v[i];
+  }
+}
+
+template <int VX>
+__device__ __forceinline__ void lds_get(const float* buf, int row, int nx, int xb, float (&v)[VX]) {
+#pragma unroll
+  for (int i = 0; i < VX; ++i) v[i] = (xb + i < nx) ? buf[row * nx + xb + i] : 0.0f;
+}
+
+template <int VX, int NW, bool FUSE>
+__global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
+  // Halo rows above the output rows: two for the steps; the fused primal of the first output
+  // row needs p_{k+2} of the row above it, which needs a third. Two below.
+  constexpr int TOP = FUSE ? 3 : 2;
+  constexpr int T = NW - 2 - TOP;
+  extern __shared__ float lds[];
+  const int nx = a.nx;
+  float* ry = lds;              // y component of r_k of each row
+  float* rqy = ry + NW * nx;    // y component of r_{k+1}
+  float* zk = rqy + NW * nx;    // z_k
+  float* zq = zk + NW * nx;     // z_{k+1}
+  float* py = zq + NW * nx;     // y component of p_{k+2} (fused primal only)
+  TvArgs bd{};
+  bd.nz = a.nz;
+  bd.ny = a.ny;
+  bd.nx = a.nx;
+  const int w = static_cast<int>(threadIdx.x) >> 6;
+  const int lane = static_cast<int>(threadIdx.x) & 63;
+  const int tile = xcd_tile();
+  const int y0 = (tile / chunks) * T;
+  const int z0 = (tile % chunks) * kPairChunk;
+  const int z1 = min(z0 + kPairChunk, a.nz);
+  const int y = y0 - TOP + w;
+  const int xb = lane * VX;
+  const bool vec = (nx % 4) == 0;
+  const int wu = w > 0 ? w - 1 : 0;       // the first and last waves read their own row: their
+  const int wd = w < NW - 1 ? w + 1 : w;  // results only feed other halo values, never a store
+  const bool out_row = w >= TOP && w < TOP + T && y < a.ny;
+  // Planes of warm-up before z0: two give p_{k+2} from plane z0 on; the fused primal needs
+  // p_{k+2} on plane z0 - 1 as well, hence one more.
+  const int pre = FUSE ? 3 : 2;
+  auto load_step1 = [&](int f, float (&R)[3][VX], float (&Pk)[3][VX], float (&B)[VX]) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      float pm[VX];
+      load_row<VX>(a.p_k + c * a.vol, bd, f, y, xb, vec, Pk[c]);
+      if (!a.first) {
+        load_row<VX>(a.p_km1 + c * a.vol, bd, f, y, xb, vec, pm);
+      } else {
+#pragma unroll
+        for (int v = 0; v < VX; ++v) pm[v] = 0.0f;
+      }
+#pragma unroll
+      for (int v = 0; v < VX; ++v) R[c][v] = a.first ? Pk[c][v] : Pk[c][v] + a.m1 * (Pk[c][v] - pm[v]);
+    }
+    load_row<VX>(a.b, bd, f, y, xb, vec, B);
+  };
+
+  // Prologue: r_k on plane z0 - pre (z component) and z_k on plane z0 - pre + 1.
+  const int zs = z0 - pre + 1;
+  float rz_m2[VX];
+  {
+    float pk[VX], pm[VX];
+    load_row<VX>(a.p_k, bd, zs - 1, y, xb, vec, pk);
+    if (!a.first) {
+      load_row<VX>(a.p_km1, bd, zs - 1, y, xb, vec, pm);
+    } else {
+#pragma unroll
+      for (int v = 0; v < VX; ++v) pm[v] = 0.0f;
+    }
+#pragma unroll
+    for (int v = 0; v < VX; ++v) rz_m2[v] = a.first ? pk[v] : pk[v] + a.m1 * (pk[v] - pm[v]);
+  }
+  float rk1[3][VX], pk1[3][VX], b1[VX], zk1[VX];
+  load_step1(zs, rk1, pk1, b1);
+  lds_put<VX>(ry, w, nx, xb, rk1[1]);
+  __syncthreads();
+  {
+    float up[VX];
+    lds_get<VX>(ry, wu, nx, xb, up);
+    pair_primal<VX>(a, zs, y, xb, rk1, rz_m2, up, b1, zk1);
+  }
+  lds_put<VX>(zk, w, nx, xb, zk1);
+  float rq0[3][VX], zq0[VX], zdn_k[VX], zdn_q[VX];
+#pragma unroll
+  for (int v = 0; v < VX; ++v) {
+    zq0[v] = 0.0f;
+    zdn_q[v] = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) rq0[c][v] = 0.0f;
+  }
+  __syncthreads();
+  lds_get<VX>(zk, wd, nx, xb, zdn_k);  // z_k of the row below on plane zs
+  // Fused primal: p_{k+2} of the previous plane (all components), its z component one plane
+  // further back, and b of the previous plane and of the current one.
+  float p2_prev[3][VX], p2z_back[VX], b_prev[VX], b0[VX];
+#pragma unroll
+  for (int v = 0; v < VX; ++v) {
+    p2z_back[v] = 0.0f;
+    b_prev[v] = 0.0f;
+    b0[v] = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) p2_prev[c][v] = 0.0f;
+  }
+  // max(b - lambda L(p_{k+2}), 0) on plane zp, from the registers above and the row above's y
+  // component in LDS (written by the previous plane's dual step).
+  auto fused_primal = [&](int zp) {
+    float up[VX], o[VX];
+    lds_get<VX>(py, wu, nx, xb, up);
+    pair_primal<VX>(a, zp, y, xb, p2_prev, p2z_back, up, b_prev, o);
+    if (out_row) {
+#pragma unroll
+      for (int v = 0; v < VX; ++v) {
+        if (a.to_index) o[v] = index_from_potential(o[v], 0.0f, a.nm2, a.inv_k02);
+      }
+      store_row<VX>(a.out, bd, zp, y, xb, vec, o);
+    }
+  };
+
+  for (int t = z0 - pre; t < z1; ++t) {
+    const int f = t + 2;
+    float rk2[3][VX], pk2[3][VX], b2[VX];
+    load_step1(f, rk2, pk2, b2);
+    lds_put<VX>(ry, w, nx, xb, rk2[1]);
+    __syncthreads();  // (1) r_k of plane f visible
+    if (FUSE && t - 1 >= z0) fused_primal(t - 1);
+    float zk2[VX];
+    {
+      float up[VX];
+      lds_get<VX>(ry, wu, nx, xb, up);
+      pair_primal<VX>(a, f, y, xb, rk2, rk1[0], up, b2, zk2);
+    }
+    // p_{k+1} and r_{k+1} on plane t + 1.
+    float pq1[3][VX], rq1[3][VX];
+    pair_dual<VX>(a, t + 1, y, xb, zk1, zk2, zdn_k, rk1, pq1);
+    if (a.p_k1 && out_row && t + 1 >= z0 && t + 1 < z1) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) store_row<VX>(a.p_k1 + c * a.vol, bd, t + 1, y, xb, vec, pq1[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+#pragma unroll
+      for (int v = 0; v < VX; ++v) rq1[c][v] = pq1[c][v] + a.m2 * (pq1[c][v] - pk1[c][v]);
+    }
+    lds_put<VX>(zk, w, nx, xb, zk2);
+    lds_put<VX>(rqy, w, nx, xb, rq1[1]);
+    __syncthreads();  // (2) z_k of plane f and r_{k+1} of plane t + 1 visible
+    lds_get<VX>(zk, wd, nx, xb, zdn_k);  // for the next plane's dual step
+    float zq1[VX];
+    {
+      float up[VX];
+      lds_get<VX>(rqy, wu, nx, xb, up);
+      pair_primal<VX>(a, t + 1, y, xb, rq1, rq0[0], up, b1, zq1);
+    }
+    lds_put<VX>(zq, w, nx, xb, zq1);
+    __syncthreads();  // (3) z_{k+1} of plane t + 1 visible
+    // p_{k+2} on plane t.
+    if (t >= z0 - (pre - 2)) {
+      float p2[3][VX];
+      pair_dual<VX>(a, t, y, xb, zq0, zq1, zdn_q, rq0, p2);
+      if (out_row && t >= z0) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) store_row<VX>(a.p_k2 + c * a.vol, bd, t, y, xb, vec, p2[c]);
+      }
+      if (FUSE) {
+        lds_put<VX>(py, w, nx, xb, p2[1]);
+#pragma unroll
+        for (int v = 0; v < VX; ++v) {
+          p2z_back[v] = p2_prev[0][v];
+#pragma unroll
+          for (int c = 0; c < 3; ++c) p2_prev[c][v] = p2[c][v];
+        }
+      }
+    }
+    lds_get<VX>(zq, wd, nx, xb, zdn_q);  // z_{k+1} of the row below on plane t + 1
+#pragma unroll
+    for (int v = 0; v < VX; ++v) {
+      b_prev[v] = b0[v];
+      b0[v] = b1[v];
+      b1[v] = b2[v];
+      zk1[v] = zk2[v];
+      zq0[v] = zq1[v];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        rk1[c][v] = rk2[c][v];
+        pk1[c][v] = pk2[c][v];
+        rq0[c][v] = rq1[c][v];
+      }
+    }
+  }
+  if (FUSE) {
+    __syncthreads();  // the last plane's y components
+    fused_primal(z1 - 1);
+  }
+}
+
 // out = max(b - lambda L(p), 0) (optionally its refractive index), same strip layout.
 template <int VX, int TY>
 __global__ void __launch_bounds__(64) k_tv_primal(TvArgs a, int chunks, float* out, int to_index,
@@ -373,42 +655,120 @@ void tv_primal(const float* b, const float* p, int nz, int ny, int nx, float lam
   check_launch("tv_primal");
 }
 
+namespace {
+
+// Rows per block of the two-step kernel: NW waves, NW - 4 output rows. Five row buffers of
+// NW x nx floats stay within 64 KiB of LDS. The pass with the fused primal carries more state
+// per lane; at 4 floats per lane it runs 8-wave blocks to keep it in registers.
+int pair_waves(int nx, bool fuse) { return nx <= 128 ? 16 : (fuse ? 8 : 12); }
+
+}  // namespace
+
+bool fgp_pair_supported(int nx) { return nx >= 1 && nx <= 256; }
+
+void fgp_pair(const float* b, const float* p_k, const float* p_km1, float* p_k1, float* p_k2,
+              int nz, int ny, int nx, float lambda, float step, float m1, bool first, float m2,
+              float* out, bool to_index, float nm2, float inv_k02, hipStream_t s) {
+  require(fgp_pair_supported(nx), "fgp_pair: rows up to 256 wide");
+  PairArgs a{b, p_k, p_km1, p_k1, p_k2, static_cast<long>(nz) * ny * nx, nz, ny, nx,
+             lambda, step, m1, first ? 1 : 0, m2, out, to_index ? 1 : 0, nm2, inv_k02};
+  const bool fuse = out != nullptr;
+  const int nw = pair_waves(nx, fuse);
+  const int rows = nw - (fuse ? 5 : 4);
+  const int chunks = (nz + kPairChunk - 1) / kPairChunk;
+  const dim3 grid(static_cast<unsigned>(((ny + rows - 1) / rows) * chunks));
+  const dim3 block(static_cast<unsigned>(nw * 64));
+  const std::size_t lds = 5 * static_cast<std::size_t>(nw) * nx * sizeof(float);
+  switch (lanes_x(nx) * 2 + (fuse ? 1 : 0)) {
+    case 2: hipLaunchKernelGGL((k_fgp_pair<1, 16, false>), grid, block, lds, s, a, chunks); break;
+    case 3: hipLaunchKernelGGL((k_fgp_pair<1, 16, true>), grid, block, lds, s, a, chunks); break;
+    case 4: hipLaunchKernelGGL((k_fgp_pair<2, 16, false>), grid, block, lds, s, a, chunks); break;
+    case 5: hipLaunchKernelGGL((k_fgp_pair<2, 16, true>), grid, block, lds, s, a, chunks); break;
+    case 8: hipLaunchKernelGGL((k_fgp_pair<4, 12, false>), grid, block, lds, s, a, chunks); break;
+    default: hipLaunchKernelGGL((k_fgp_pair<4, 8, true>), grid, block, lds, s, a, chunks); break;
+  }
+  check_launch("fgp_pair");
+}
+
 TvState::TvState(int nz, int ny, int nx)
     : nz_(nz), ny_(ny), nx_(nx), vol_(static_cast<long>(nz) * ny * nx),
       p_{DeviceBuffer<float>(3 * vol_), DeviceBuffer<float>(3 * vol_),
-         DeviceBuffer<float>(3 * vol_)} {}
+         DeviceBuffer<float>(3 * vol_), DeviceBuffer<float>(3 * vol_)} {}
 
 void TvState::reset(hipStream_t s) {
   fill(p_[0].get(), 3 * vol_, 0.0f, s);
   cur_ = 0;
 }
 
-std::size_t TvState::bytes() const { return p_[0].bytes() + p_[1].bytes() + p_[2].bytes(); }
+std::size_t TvState::bytes() const {
+  return p_[0].bytes() + p_[1].bytes() + p_[2].bytes() + p_[3].bytes();
+}
 
 // tv_prox of the reference (src/cpu/tv.cpp:7-44): r <- p, `inner` FGP steps, out = primal(p).
-// The dual iterates rotate through three buffers: step k reads p_k and p_{k-1} and writes p_{k+1}
-// to the third; r is formed inside the step. The prox leaves p in buffer cur_ for the next call.
+// The dual iterates rotate through four buffers: a pass reads p_k and p_{k-1} (r is formed
+// inside it) and writes p_{k+1} (and p_{k+2} for a two-step pass) to buffers nobody reads in
+// that pass. Steps go in pairs where the row width allows, with an odd last step on its own; the
+// momentum sequence is that of single steps. When the last pass is a pair it also writes the
+// primal output (and skips p_{k+1}, which the next call does not read: its first step uses
+// r = p). The prox leaves p in buffer cur_ for the next call.
 void TvState::prox(const float* b, float* out, float lambda, int inner, hipStream_t s,
                    bool to_index, float nm2, float inv_k02, Event* t0, Event* t1) {
   const float step = 1.0f / (12.0f * lambda);
+  const bool pairs = fused_ && fgp_pair_supported(nx_);
+  bool fused_primal = false;
   float t = 1.0f;
   float m_prev = 0.0f;
   int cur = cur_;
-  int prev = (cur_ + 2) % 3;
+  int prev = (cur_ + 1) % 4;
+  launches_ = 0;
+  const auto free_buffers = [&](int& f1, int& f2) {
+    f1 = -1;
+    f2 = -1;
+    for (int i = 0; i < 4; ++i) {
+      if (i == cur || i == prev) continue;
+      if (f1 < 0) {
+        f1 = i;
+      } else {
+        f2 = i;
+      }
+    }
+  };
   if (t0) t0->record(s);
-  for (int k = 0; k < inner; ++k) {
-    const int next = 3 - cur - prev;
-    fgp_step(b, p_[cur].get(), p_[prev].get(), p_[next].get(), nz_, ny_, nx_, lambda, step,
-             m_prev, k == 0, s);
-    const float tn = fista_t(t);
-    m_prev = (t - 1.0f) / tn;
-    t = tn;
-    prev = cur;
-    cur = next;
+  for (int k = 0; k < inner;) {
+    int f1 = 0, f2 = 0;
+    free_buffers(f1, f2);
+    if (pairs && k + 1 < inner) {
+      const float m1 = m_prev;
+      float tn = fista_t(t);
+      const float m2 = (t - 1.0f) / tn;
+      t = tn;
+      // The last pass skips p_{k+1} (the next prox starts from p alone) and writes the primal.
+      const bool last = k + 2 == inner;
+      fgp_pair(b, p_[cur].get(), p_[prev].get(), last ? nullptr : p_[f1].get(), p_[f2].get(),
+               nz_, ny_, nx_, lambda, step, m1, k == 0, m2, last ? out : nullptr, to_index, nm2,
+               inv_k02, s);
+      fused_primal = last;
+      tn = fista_t(t);
+      m_prev = (t - 1.0f) / tn;
+      t = tn;
+      prev = f1;
+      cur = f2;
+      k += 2;
+    } else {
+      fgp_step(b, p_[cur].get(), p_[prev].get(), p_[f1].get(), nz_, ny_, nx_, lambda, step,
+               m_prev, k == 0, s);
+      const float tn = fista_t(t);
+      m_prev = (t - 1.0f) / tn;
+      t = tn;
+      prev = cur;
+      cur = f1;
+      k += 1;
+    }
+    ++launches_;
   }
   if (t1) t1->record(s);
   cur_ = cur;
-  tv_primal(b, p_[cur_].get(), nz_, ny_, nx_, lambda, out, to_index, nm2, inv_k02, s);
+  if (!fused_primal) tv_primal(b, p_[cur_].get(), nz_, ny_, nx_, lambda, out, to_index, nm2, inv_k02, s);
 }
 
 }  // namespace qh

@@ -19,12 +19,22 @@ void fgp_step(const float* b, const float* p_in, const float* p_prev, float* p_o
               int ny, int nx, float lambda, float step, float momentum, bool first,
               hipStream_t s);
 
+// Two FGP steps in one pass (rows up to 256 wide): from p_k, p_{k-1} to p_{k+1} and p_{k+2},
+// bit for bit what two fgp_step calls with momenta m1 then m2 give. The four buffers are
+// distinct; p_k1 may be null (not written). With out, the pass also writes tv_primal of
+// p_{k+2} (optionally the index), as a following tv_primal call would.
+bool fgp_pair_supported(int nx);
+void fgp_pair(const float* b, const float* p_k, const float* p_km1, float* p_k1, float* p_k2,
+              int nz, int ny, int nx, float lambda, float step, float m1, bool first, float m2,
+              float* out, bool to_index, float nm2, float inv_k02, hipStream_t s);
+
 // out = max(b - lambda L(p), 0); with to_index the refractive index of that potential.
 void tv_primal(const float* b, const float* p, int nz, int ny, int nx, float lambda, float* out,
                bool to_index, float nm2, float inv_k02, hipStream_t s);
 
-// Dual state of the TV prox, kept across calls (FISTA warm start): three rotating dual buffers
-// of three components each.
+// Dual state of the TV prox, kept across calls (FISTA warm start): four rotating dual buffers
+// of three components each. Steps run two per pass where the row width allows (set_fused(false)
+// runs them one per pass, for comparisons).
 class TvState {
  public:
   TvState(int nz, int ny, int nx);
@@ -35,12 +45,17 @@ class TvState {
             bool to_index = false, float nm2 = 0.0f, float inv_k02 = 0.0f, Event* t0 = nullptr,
             Event* t1 = nullptr);
   std::size_t bytes() const;
+  void set_fused(bool on) { fused_ = on; }
+  // Kernel passes of the last prox call.
+  int launches() const { return launches_; }
 
  private:
   int nz_, ny_, nx_;
   long vol_;
-  DeviceBuffer<float> p_[3];
+  DeviceBuffer<float> p_[4];
   int cur_ = 0;
+  bool fused_ = true;
+  int launches_ = 0;
 };
 
 }  // namespace qh

@@ -26,7 +26,8 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import Geometry, circular_scan, find_sideband
+from . import Geometry, circular_scan, find_sideband, hip_available
+from . import estimate_illuminations as device_estimate
 from .phantom import Ellipsoid, Sphere, forward_fields, synthesize_holograms
 
 WAVELENGTH = 0.532
@@ -117,3 +118,47 @@ def bead(n: int = 128, angles: int = 49, seed: int = 0) -> Workload:
     bg = _noisy(np.ones_like(ratio), illum, holo_geom, seed + 1)
     return Workload("bead", holo, bg, (70.5, 70.5), 32.0, n, geom, holo_geom, illum, illum,
                     shapes)
+
+
+def random_cell(n: int, seed: int) -> list:
+    """3-8 spheres with n ~ U(1.345, 1.395), each either inside an earlier one or disjoint from
+    all of them (the painter's-order rule of the synthesiser), by seeded rejection sampling in
+    a field of view of n voxels of 0.11 um."""
+    rng = np.random.default_rng(seed)
+    half = 0.4 * n * 2 * HOLO_PIXEL
+    count = int(rng.integers(3, 9))
+    placed = []
+    while len(placed) < count:
+        r = float(rng.uniform(0.08, 0.25) * half)
+        c = tuple(float(v) for v in rng.uniform(-half + r, half - r, 3))
+        ok = True
+        for (pc, pr, _) in placed:
+            d = math.dist(c, pc)
+            if not (d > r + pr + 0.2 or d + r < pr - 0.2):
+                ok = False
+                break
+        if ok:
+            placed.append((c, r, float(rng.uniform(1.345, 1.395))))
+    # Larger spheres first so that nested ones are painted over their parents.
+    placed.sort(key=lambda p: -p[1])
+    return [Sphere(z=c[0], y=c[1], x=c[2], radius=r, n=nn) for c, r, nn in placed]
+
+
+def mixed_cell(n: int, angles: int, seed: int = 0) -> Workload:
+    """A random cell of ``random_cell`` at side n (holograms 2n at 0.055 um), angles from the
+    backgrounds, crop radius ceil((NA / lambda) / df) + 1."""
+    holo_geom = Geometry(WAVELENGTH, N_MEDIUM, NA, HOLO_PIXEL)
+    geom = Geometry(WAVELENGTH, N_MEDIUM, NA, 2 * HOLO_PIXEL)
+    illum = circular_scan(angles, NA_ILL, geom)
+    shapes = random_cell(n, seed)
+    ratio = forward_fields(shapes, illum, holo_geom, 2 * n)
+    holo = _noisy(ratio, illum, holo_geom, seed)
+    bg = _noisy(np.ones_like(ratio), illum, holo_geom, seed + 1)
+    if hip_available():
+        c, est = device_estimate(bg, holo_geom)
+    else:
+        c, est = estimate_illuminations(bg, 2 * n, HOLO_PIXEL)
+    df = 1.0 / (2 * n * HOLO_PIXEL)
+    radius = float(math.ceil(NA / WAVELENGTH / df) + 1)
+    return Workload(f"mixed_cell_{n}", holo, bg, (float(c.iy), float(c.ix)), radius, n, geom,
+                    holo_geom, est, illum, shapes)

@@ -178,9 +178,30 @@ def test_tv_prox_partial_tiles(device, oracle):
     lam = 0.02
     want = oracle.tv_prox(b, lam, 10)
     got = np.empty_like(b)
-    _native.check(_native.lib().qh_hip_tv_prox(_native.ptr(b), 62, 60, 60, lam, 10,
+    _native.check(_native.lib().qh_hip_tv_prox(_native.ptr(b), 62, 60, 60, lam, 10, 1,
                                                _native.ptr(got)))
     assert np.max(np.abs(got - want)) < 1e-6
+
+
+@pytest.mark.parametrize("shape,inner", [((62, 60, 60), 10), ((40, 33, 130), 7), ((70, 128, 100), 4),
+                                         ((33, 77, 256), 5), ((256, 256, 256), 10),
+                                         ((20, 30, 300), 3)])
+def test_tv_two_step_passes_equal_single_steps(device, shape, inner):
+    """Two FGP steps per pass give the bits of one step per pass: partial row tiles and z
+    chunks, odd inner counts (a single last step), row widths for 1, 2 and 4 floats per lane,
+    the full 256^3 volume, and a width the paired kernel does not take (single steps)."""
+    from qoraholo_amd import _native
+
+    rng = np.random.default_rng(shape[0])
+    b = (rng.standard_normal(shape) * 0.1 + 0.05).astype(np.float32)
+    print(f"tv pair shape={shape} inner={inner}", flush=True)
+    outs = []
+    for fused in (1, 0):
+        got = np.empty_like(b)
+        _native.check(_native.lib().qh_hip_tv_prox(_native.ptr(b), *shape, 0.02, inner, fused,
+                                                   _native.ptr(got)))
+        outs.append(got)
+    assert np.array_equal(outs[0], outs[1])
 
 
 def test_tv_timings_count_fgp_steps(device, qh, rytov_fields):
