@@ -384,6 +384,7 @@ class Pipeline:
                                              upload_chunks)
         handle = ctypes.c_void_p()
         _native.check(_native.lib().qh_hip_pipeline_create(ctypes.byref(o), ctypes.byref(handle)))
+        self._destroy = _native.lib().qh_hip_pipeline_destroy
         self._handle = handle
         self.count = count
         self.hologram_size = hologram_size
@@ -393,8 +394,10 @@ class Pipeline:
         self.output_shape = (nzc.value, nyc.value, nxc.value)
 
     def __del__(self):
-        if getattr(self, "_handle", None):
-            _native.lib().qh_hip_pipeline_destroy(self._handle)
+        # The destroy function is held by the object: module globals may already be gone when
+        # the interpreter shuts down.
+        if getattr(self, "_handle", None) and getattr(self, "_destroy", None):
+            self._destroy(self._handle)
             self._handle = None
 
     def _host(self, addr, shape):
@@ -488,13 +491,16 @@ class Server:
         handle = ctypes.c_void_p()
         _native.check(_native.lib().qh_hip_server_create(int(budget_bytes), int(max_slots),
                                                          ctypes.byref(handle)))
+        self._destroy = _native.lib().qh_hip_server_destroy
         self._handle = handle
         self._geometries = {}
         self._pending = {}
 
     def __del__(self):
-        if getattr(self, "_handle", None):
-            _native.lib().qh_hip_server_destroy(self._handle)
+        # The destroy function is held by the object: module globals may already be gone when
+        # the interpreter shuts down.
+        if getattr(self, "_handle", None) and getattr(self, "_destroy", None):
+            self._destroy(self._handle)
             self._handle = None
 
     def add_geometry(self, count: int, hologram_size: int, center, illuminations,

@@ -181,6 +181,7 @@ class BpmPlan:
         _native.check(_native.lib().qh_hip_bpm_plan_create(
             self.nz, self.n, _native.ptr(il), self.count, ctypes.byref(geometry._c()),
             ctypes.byref(_opts(obliquity, chunk, checkpoint)), ctypes.byref(h)))
+        self._destroy = _native.lib().qh_hip_bpm_plan_destroy
         self._h = h
         phase, lip, ch, nbytes = ctypes.c_float(), ctypes.c_float(), ctypes.c_int(), ctypes.c_size_t()
         _native.check(_native.lib().qh_hip_bpm_plan_info(h, ctypes.byref(phase), ctypes.byref(lip),
@@ -189,8 +190,10 @@ class BpmPlan:
         self._bytes = nbytes
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            _native.lib().qh_hip_bpm_plan_destroy(self._h)
+        # The destroy function is held by the object: module globals may already be gone when
+        # the interpreter shuts down.
+        if getattr(self, "_h", None) and getattr(self, "_destroy", None):
+            self._destroy(self._h)
             self._h = None
 
     @property

@@ -261,8 +261,7 @@ __global__ void __launch_bounds__(64) k_fgp_step(TvArgs a, int chunks) {
 
 // ---- two FGP steps per pass ----
 //
-// A block of NW waves owns NW - 4 output rows (NW - 5 when the pass also writes the primal);
-// wave w holds row y0 - 2 + w (y0 - 3 + w) across its 64 lanes
+// A block of NW waves owns NW - 4 output rows; wave w holds row y0 - 2 + w across its 64 lanes
 // (VX floats each) and the block marches z through a chunk of planes. Per plane t the waves
 // load b, p_k, p_{k-1} of plane t + 2, form r_k and z_k there, the dual p_{k+1} of plane t + 1
 // and its r_{k+1}, z_{k+1} of plane t + 1, and p_{k+2} of plane t. y neighbours come through
@@ -286,12 +285,6 @@ struct PairArgs {
   float m1;  // momentum of r_k = p_k + m1 (p_k - p_{k-1})
   int first;
   float m2;  // momentum of r_{k+1} = p_{k+1} + m2 (p_{k+1} - p_k)
-  // Last pass of a prox: p_k1 may be null (p_{k+1} is not needed afterwards) and out, when
-  // given, receives max(b - lambda L(p_{k+2}), 0) (or its refractive index with to_index).
-  float* out;
-  int to_index;
-  float nm2;
-  float inv_k02;
 };
 
 // z = max(b - lambda L(r), 0) on one row of plane zp (the expression order of primal_plane).
@@ -354,19 +347,16 @@ __device__ __forceinline__ void lds_get(const float* buf, int row, int nx, int x
   for (int i = 0; i < VX; ++i) v[i] = (xb + i < nx) ? buf[row * nx + xb + i] : 0.0f;
 }
 
-template <int VX, int NW, bool FUSE>
+template <int VX, int NW>
 __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
-  // Halo rows above the output rows: two for the steps; the fused primal of the first output
-  // row needs p_{k+2} of the row above it, which needs a third. Two below.
-  constexpr int TOP = FUSE ? 3 : 2;
-  constexpr int T = NW - 2 - TOP;
+  constexpr int TOP = 2;  // halo rows above the output rows (and two below)
+  constexpr int T = NW - 4;
   extern __shared__ float lds[];
   const int nx = a.nx;
   float* ry = lds;              // y component of r_k of each row
   float* rqy = ry + NW * nx;    // y component of r_{k+1}
   float* zk = rqy + NW * nx;    // z_k
   float* zq = zk + NW * nx;     // z_{k+1}
-  float* py = zq + NW * nx;     // y component of p_{k+2} (fused primal only)
   TvArgs bd{};
   bd.nz = a.nz;
   bd.ny = a.ny;
@@ -383,9 +373,7 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
   const int wu = w > 0 ? w - 1 : 0;       // the first and last waves read their own row: their
   const int wd = w < NW - 1 ? w + 1 : w;  // results only feed other halo values, never a store
   const bool out_row = w >= TOP && w < TOP + T && y < a.ny;
-  // Planes of warm-up before z0: two give p_{k+2} from plane z0 on; the fused primal needs
-  // p_{k+2} on plane z0 - 1 as well, hence one more.
-  const int pre = FUSE ? 3 : 2;
+  const int pre = 2;  // planes of warm-up before z0
   auto load_step1 = [&](int f, float (&R)[3][VX], float (&Pk)[3][VX], float (&B)[VX]) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -438,31 +426,6 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
   }
   __syncthreads();
   lds_get<VX>(zk, wd, nx, xb, zdn_k);  // z_k of the row below on plane zs
-  // Fused primal: p_{k+2} of the previous plane (all components), its z component one plane
-  // further back, and b of the previous plane and of the current one.
-  float p2_prev[3][VX], p2z_back[VX], b_prev[VX], b0[VX];
-#pragma unroll
-  for (int v = 0; v < VX; ++v) {
-    p2z_back[v] = 0.0f;
-    b_prev[v] = 0.0f;
-    b0[v] = 0.0f;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) p2_prev[c][v] = 0.0f;
-  }
-  // max(b - lambda L(p_{k+2}), 0) on plane zp, from the registers above and the row above's y
-  // component in LDS (written by the previous plane's dual step).
-  auto fused_primal = [&](int zp) {
-    float up[VX], o[VX];
-    lds_get<VX>(py, wu, nx, xb, up);
-    pair_primal<VX>(a, zp, y, xb, p2_prev, p2z_back, up, b_prev, o);
-    if (out_row) {
-#pragma unroll
-      for (int v = 0; v < VX; ++v) {
-        if (a.to_index) o[v] = index_from_potential(o[v], 0.0f, a.nm2, a.inv_k02);
-      }
-      store_row<VX>(a.out, bd, zp, y, xb, vec, o);
-    }
-  };
 
   for (int t = z0 - pre; t < z1; ++t) {
     const int f = t + 2;
@@ -470,7 +433,6 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
     load_step1(f, rk2, pk2, b2);
     lds_put<VX>(ry, w, nx, xb, rk2[1]);
     __syncthreads();  // (1) r_k of plane f visible
-    if (FUSE && t - 1 >= z0) fused_primal(t - 1);
     float zk2[VX];
     {
       float up[VX];
@@ -502,28 +464,17 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
     lds_put<VX>(zq, w, nx, xb, zq1);
     __syncthreads();  // (3) z_{k+1} of plane t + 1 visible
     // p_{k+2} on plane t.
-    if (t >= z0 - (pre - 2)) {
+    if (t >= z0) {
       float p2[3][VX];
       pair_dual<VX>(a, t, y, xb, zq0, zq1, zdn_q, rq0, p2);
-      if (out_row && t >= z0) {
+      if (out_row) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) store_row<VX>(a.p_k2 + c * a.vol, bd, t, y, xb, vec, p2[c]);
-      }
-      if (FUSE) {
-        lds_put<VX>(py, w, nx, xb, p2[1]);
-#pragma unroll
-        for (int v = 0; v < VX; ++v) {
-          p2z_back[v] = p2_prev[0][v];
-#pragma unroll
-          for (int c = 0; c < 3; ++c) p2_prev[c][v] = p2[c][v];
-        }
       }
     }
     lds_get<VX>(zq, wd, nx, xb, zdn_q);  // z_{k+1} of the row below on plane t + 1
 #pragma unroll
     for (int v = 0; v < VX; ++v) {
-      b_prev[v] = b0[v];
-      b0[v] = b1[v];
       b1[v] = b2[v];
       zk1[v] = zk2[v];
       zq0[v] = zq1[v];
@@ -534,10 +485,6 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
         rq0[c][v] = rq1[c][v];
       }
     }
-  }
-  if (FUSE) {
-    __syncthreads();  // the last plane's y components
-    fused_primal(z1 - 1);
   }
 }
 
@@ -657,10 +604,9 @@ void tv_primal(const float* b, const float* p, int nz, int ny, int nx, float lam
 
 namespace {
 
-// Rows per block of the two-step kernel: NW waves, NW - 4 output rows. Five row buffers of
-// NW x nx floats stay within 64 KiB of LDS. The pass with the fused primal carries more state
-// per lane; at 4 floats per lane it runs 8-wave blocks to keep it in registers.
-int pair_waves(int nx, bool fuse) { return nx <= 128 ? 16 : (fuse ? 8 : 12); }
+// Rows per block of the two-step kernel: NW waves, NW - 4 output rows. Four row buffers of
+// NW x nx floats stay within 64 KiB of LDS.
+int pair_waves(int nx) { return nx <= 128 ? 16 : 12; }
 
 }  // namespace
 
@@ -668,24 +614,20 @@ bool fgp_pair_supported(int nx) { return nx >= 1 && nx <= 256; }
 
 void fgp_pair(const float* b, const float* p_k, const float* p_km1, float* p_k1, float* p_k2,
               int nz, int ny, int nx, float lambda, float step, float m1, bool first, float m2,
-              float* out, bool to_index, float nm2, float inv_k02, hipStream_t s) {
+              hipStream_t s) {
   require(fgp_pair_supported(nx), "fgp_pair: rows up to 256 wide");
   PairArgs a{b, p_k, p_km1, p_k1, p_k2, static_cast<long>(nz) * ny * nx, nz, ny, nx,
-             lambda, step, m1, first ? 1 : 0, m2, out, to_index ? 1 : 0, nm2, inv_k02};
-  const bool fuse = out != nullptr;
-  const int nw = pair_waves(nx, fuse);
-  const int rows = nw - (fuse ? 5 : 4);
+             lambda, step, m1, first ? 1 : 0, m2};
+  const int nw = pair_waves(nx);
+  const int rows = nw - 4;
   const int chunks = (nz + kPairChunk - 1) / kPairChunk;
   const dim3 grid(static_cast<unsigned>(((ny + rows - 1) / rows) * chunks));
   const dim3 block(static_cast<unsigned>(nw * 64));
-  const std::size_t lds = 5 * static_cast<std::size_t>(nw) * nx * sizeof(float);
-  switch (lanes_x(nx) * 2 + (fuse ? 1 : 0)) {
-    case 2: hipLaunchKernelGGL((k_fgp_pair<1, 16, false>), grid, block, lds, s, a, chunks); break;
-    case 3: hipLaunchKernelGGL((k_fgp_pair<1, 16, true>), grid, block, lds, s, a, chunks); break;
-    case 4: hipLaunchKernelGGL((k_fgp_pair<2, 16, false>), grid, block, lds, s, a, chunks); break;
-    case 5: hipLaunchKernelGGL((k_fgp_pair<2, 16, true>), grid, block, lds, s, a, chunks); break;
-    case 8: hipLaunchKernelGGL((k_fgp_pair<4, 12, false>), grid, block, lds, s, a, chunks); break;
-    default: hipLaunchKernelGGL((k_fgp_pair<4, 8, true>), grid, block, lds, s, a, chunks); break;
+  const std::size_t lds = 4 * static_cast<std::size_t>(nw) * nx * sizeof(float);
+  switch (lanes_x(nx)) {
+    case 1: hipLaunchKernelGGL((k_fgp_pair<1, 16>), grid, block, lds, s, a, chunks); break;
+    case 2: hipLaunchKernelGGL((k_fgp_pair<2, 16>), grid, block, lds, s, a, chunks); break;
+    default: hipLaunchKernelGGL((k_fgp_pair<4, 12>), grid, block, lds, s, a, chunks); break;
   }
   check_launch("fgp_pair");
 }
@@ -708,14 +650,13 @@ std::size_t TvState::bytes() const {
 // The dual iterates rotate through four buffers: a pass reads p_k and p_{k-1} (r is formed
 // inside it) and writes p_{k+1} (and p_{k+2} for a two-step pass) to buffers nobody reads in
 // that pass. Steps go in pairs where the row width allows, with an odd last step on its own; the
-// momentum sequence is that of single steps. When the last pass is a pair it also writes the
-// primal output (and skips p_{k+1}, which the next call does not read: its first step uses
-// r = p). The prox leaves p in buffer cur_ for the next call.
+// momentum sequence is that of single steps. A last pass that is a pair skips p_{k+1}, which the
+// next call does not read (its first step uses r = p). The prox leaves p in buffer cur_ for the
+// next call.
 void TvState::prox(const float* b, float* out, float lambda, int inner, hipStream_t s,
                    bool to_index, float nm2, float inv_k02, Event* t0, Event* t1) {
   const float step = 1.0f / (12.0f * lambda);
   const bool pairs = fused_ && fgp_pair_supported(nx_);
-  bool fused_primal = false;
   float t = 1.0f;
   float m_prev = 0.0f;
   int cur = cur_;
@@ -742,12 +683,10 @@ void TvState::prox(const float* b, float* out, float lambda, int inner, hipStrea
       float tn = fista_t(t);
       const float m2 = (t - 1.0f) / tn;
       t = tn;
-      // The last pass skips p_{k+1} (the next prox starts from p alone) and writes the primal.
+      // The last pass skips p_{k+1}: the next prox starts from p alone.
       const bool last = k + 2 == inner;
       fgp_pair(b, p_[cur].get(), p_[prev].get(), last ? nullptr : p_[f1].get(), p_[f2].get(),
-               nz_, ny_, nx_, lambda, step, m1, k == 0, m2, last ? out : nullptr, to_index, nm2,
-               inv_k02, s);
-      fused_primal = last;
+               nz_, ny_, nx_, lambda, step, m1, k == 0, m2, s);
       tn = fista_t(t);
       m_prev = (t - 1.0f) / tn;
       t = tn;
@@ -768,7 +707,7 @@ void TvState::prox(const float* b, float* out, float lambda, int inner, hipStrea
   }
   if (t1) t1->record(s);
   cur_ = cur;
-  if (!fused_primal) tv_primal(b, p_[cur_].get(), nz_, ny_, nx_, lambda, out, to_index, nm2, inv_k02, s);
+  tv_primal(b, p_[cur_].get(), nz_, ny_, nx_, lambda, out, to_index, nm2, inv_k02, s);
 }
 
 }  // namespace qh

@@ -21,12 +21,11 @@ void fgp_step(const float* b, const float* p_in, const float* p_prev, float* p_o
 
 // Two FGP steps in one pass (rows up to 256 wide): from p_k, p_{k-1} to p_{k+1} and p_{k+2},
 // bit for bit what two fgp_step calls with momenta m1 then m2 give. The four buffers are
-// distinct; p_k1 may be null (not written). With out, the pass also writes tv_primal of
-// p_{k+2} (optionally the index), as a following tv_primal call would.
+// distinct; p_k1 may be null (not written).
 bool fgp_pair_supported(int nx);
 void fgp_pair(const float* b, const float* p_k, const float* p_km1, float* p_k1, float* p_k2,
               int nz, int ny, int nx, float lambda, float step, float m1, bool first, float m2,
-              float* out, bool to_index, float nm2, float inv_k02, hipStream_t s);
+              hipStream_t s);
 
 // out = max(b - lambda L(p), 0); with to_index the refractive index of that potential.
 void tv_primal(const float* b, const float* p, int nz, int ny, int nx, float lambda, float* out,
