@@ -272,6 +272,60 @@ def served_mixed(qh, rates=(0.25, 0.5, 1.0, 2.0), per_rate=16, seed=0):
     return out
 
 
+def served_bpm(qh, sizes=((192, 49), (384, 81)), iterations=40, seed=0):
+    """Config 5: BPM inversion requests from host arrays, one at a time. Each request is the
+    reference's recipe (examples/bpm_figure.py:74-88): Rytov of the field ratios, 20 GP
+    iterations for the start, then 40 full-batch FISTA iterations of the BPM with the TV prox
+    (weight 1e-2, 10 inner steps). Sizes 192^2 x 192 with 49 illuminations and 384^2 x 384 with
+    81; random cells at 0.2 um. Per iteration, the reference's traffic model of a gradient
+    (forward, 15/16 recomputation from checkpoints, adjoint) is the algorithmic figure."""
+    from qoraholo_amd import workloads
+
+    out = []
+    for n, a in sizes:
+        t0 = time.perf_counter()
+        req = workloads.bpm_request(n, a, seed=seed)
+        log(f"served bpm {n}^3 x {a}: request built in {time.perf_counter() - t0:.1f} s")
+        g, il, f = req.geometry, req.illuminations, req.fields
+        flat = np.full((n, n, n), np.float32(g.n_medium))
+        qh.bpm_reconstruct(f, il, g, flat, iterations=2, tv_weight=TV_WEIGHT)  # warm-up
+        s0 = time.perf_counter()
+        ry = qh.rytov(f)
+        s1 = time.perf_counter()
+        start = qh.reconstruct(ry, il, g, iterations=ITERATIONS)
+        s2 = time.perf_counter()
+        ri, loss = qh.bpm_reconstruct(f, il, g, start, iterations=iterations,
+                                      tv_weight=TV_WEIGHT, tv_inner=INNER, return_loss=True)
+        s3 = time.perf_counter()
+        # One more solve of 2 iterations separates the per-iteration cost from the fixed one.
+        qh.bpm_reconstruct(f, il, g, start, iterations=2, tv_weight=TV_WEIGHT, tv_inner=INNER)
+        s4 = time.perf_counter()
+        per_iter = 1000.0 * ((s3 - s2) - (s4 - s3)) / (iterations - 2)
+        grad_alg = int((88 * (1 + 15 / 16) + 100) * n * n * a * n)
+        truth = req.truth.astype(np.float64)
+        row = {
+            "workload": f"random cell {n}^2 x {n} at 0.2 um, {a} illuminations NA 0.9, "
+                        f"Rytov + GP {ITERATIONS} start, {iterations} FISTA iterations, "
+                        f"TV {TV_WEIGHT} x {INNER}",
+            "request_ms": round(1000.0 * (s3 - s0), 1),
+            "rytov_ms": round(1000.0 * (s1 - s0), 1),
+            "gp_start_ms": round(1000.0 * (s2 - s1), 1),
+            "bpm_ms": round(1000.0 * (s3 - s2), 1),
+            "iteration_ms": round(per_iter, 2),
+            "iteration_roofline": {"bound": "hbm", "algorithmic_bytes": grad_alg,
+                                   "achieved": round(grad_alg / (per_iter * 1e-3) / 1e9, 1),
+                                   "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                   "frac": round(grad_alg / (per_iter * 1e-3) / 1e9
+                                                 / HBM_PEAK_GBS, 4)},
+            "loss_first_last": [float(loss[0]), float(loss[-1])],
+            "rmse_start_end": [float(np.sqrt(np.mean((start - truth) ** 2))),
+                               float(np.sqrt(np.mean((ri - truth) ** 2)))],
+        }
+        log(f"served bpm {n}^3 x {a}: {row}")
+        out.append(row)
+    return {"requests": out}
+
+
 def pct(v, q):
     return round(float(np.percentile(v, q)), 3)
 
@@ -380,6 +434,8 @@ def main() -> int:
     ap.add_argument("--served-mixed", action="store_true",
                     help="run only the mixed served stream (128^3/256^3/384^3, GP and TV, "
                          "Poisson arrivals at 0.25x-2x the plain capacity) and print its JSON")
+    ap.add_argument("--served-bpm", action="store_true",
+                    help="only the served BPM inversion requests (192^3 x 49, 384^3 x 81)")
     ap.add_argument("--fgp-traffic", action="store_true",
                     help="run only 5 FGP steps at 256^3 and a 256 MiB device copy, for the "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE runs that calibrate HBM traffic")
@@ -402,6 +458,9 @@ def main() -> int:
         raise SystemExit("no HIP device or libqoraholo_hip.so missing")
     if args.served_mixed:
         print(json.dumps(served_mixed(qh)), flush=True)
+        return 0
+    if args.served_bpm:
+        print(json.dumps(served_bpm(qh)), flush=True)
         return 0
     if args.bpm_only:
         print(json.dumps(bpm_leg(qh, torch, cpu=not args.no_cpu_baseline, repeats=args.steps)),

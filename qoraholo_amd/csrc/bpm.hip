@@ -335,9 +335,13 @@ template <bool ROWS>
 void launch(Kind k, Args a, hipStream_t s) {
   switch (a.g.n) {
     case 32: launch_reg<ROWS, 8, 4>(k, a, s); break;
+    case 48: launch_reg<ROWS, 8, 6>(k, a, s); break;
     case 64: launch_reg<ROWS, 8, 8>(k, a, s); break;
+    case 96: launch_reg<ROWS, 16, 6>(k, a, s); break;
     case 128: launch_reg<ROWS, 16, 8>(k, a, s); break;
+    case 192: launch_reg<ROWS, 16, 12>(k, a, s); break;
     case 256: launch_reg<ROWS, 16, 16>(k, a, s); break;
+    case 384: launch_reg<ROWS, 32, 12>(k, a, s); break;
     case 512: launch_reg<ROWS, 32, 16>(k, a, s); break;
     default: {
       using E = LdsLines<ROWS>;

@@ -279,7 +279,7 @@ __global__ void __launch_bounds__(kThreads) k_z_roundtrip_restore(ZArgs a) {
   }
 }
 
-// ---- four-step kernels for lengths 32 .. 512 ----
+// ---- four-step kernels for lengths 2^a 3^b from 32 to 768 (N1 a power of two >= N2) ----
 
 template <int N1, int N2>
 struct Shape4 {
@@ -474,10 +474,15 @@ template <template <int, int> class F, class... Args>
 bool with_four_step(int n, Args&&... args) {
   switch (n) {
     case 32: F<8, 4>::run(args...); return true;
+    case 48: F<8, 6>::run(args...); return true;
     case 64: F<8, 8>::run(args...); return true;
+    case 96: F<16, 6>::run(args...); return true;
     case 128: F<16, 8>::run(args...); return true;
+    case 192: F<16, 12>::run(args...); return true;
     case 256: F<16, 16>::run(args...); return true;
+    case 384: F<32, 12>::run(args...); return true;
     case 512: F<32, 16>::run(args...); return true;
+    case 768: F<32, 24>::run(args...); return true;
     default: return false;
   }
 }

@@ -5,8 +5,8 @@
 // Thread n1 of a line holds its N2 samples x[n1 + N1 n2] in registers and transforms them
 // (step 1), multiplies by W_N^(n1 k2), and the block transposes through LDS once so that
 // thread k2 holds Y[., k2] and does the N1-point transform (step 2), ending with
-// X[N2 k1 + k2] for all k1 in its registers. The small DFTs (2, 4, 8, 16, 32) run entirely in
-// registers with constant twiddles. Compared with radix passes between LDS buffers this moves
+// X[N2 k1 + k2] for all k1 in its registers. The small DFTs (2, 3, 4, 6, 8, 12, 16, 24, 32) run
+// entirely in registers with constant twiddles. Compared with radix passes between LDS buffers this moves
 // each value through LDS once per transform.
 #pragma once
 
@@ -16,7 +16,7 @@
 
 namespace qh {
 
-// exp(-2 pi i j / 16) and exp(-2 pi i j / 32), rounded from double.
+// exp(-2 pi i j / 16), exp(-2 pi i j / 32) and exp(-2 pi i j / 24), rounded from double.
 __device__ constexpr float kW16[16][2] = {
     {1.0f, 0.0f}, {0.92387953251128673848f, -0.38268343236508978178f},
     {0.70710678118654757274f, -0.70710678118654746172f}, {0.38268343236508983729f, -0.92387953251128673848f},
@@ -45,13 +45,32 @@ __device__ constexpr float kW32[32][2] = {
     {0.70710678118654735069f, 0.70710678118654768376f}, {0.83146961230254523567f, 0.55557023301960217765f},
     {0.92387953251128651644f, 0.38268343236509039240f}, {0.98078528040323031956f, 0.19509032201612871993f}};
 
+__device__ constexpr float kW24[24][2] = {
+    {1.0f, 0.0f}, {0.96592582628906831221f, -0.25881904510252073948f},
+    {0.86602540378443870761f, -0.49999999999999994449f}, {0.70710678118654757274f, -0.70710678118654746172f},
+    {0.50000000000000011102f, -0.86602540378443859659f}, {0.25881904510252073948f, -0.96592582628906831221f},
+    {0.0f, -1.0f}, {-0.25881904510252062845f, -0.96592582628906831221f},
+    {-0.49999999999999977796f, -0.86602540378443870761f}, {-0.70710678118654746172f, -0.70710678118654757274f},
+    {-0.86602540378443870761f, -0.49999999999999994449f}, {-0.96592582628906820119f, -0.25881904510252101703f},
+    {-1.0f, 0.0f}, {-0.96592582628906831221f, 0.25881904510252079499f},
+    {-0.86602540378443881863f, 0.49999999999999972244f}, {-0.7071067811865479058f, 0.70710678118654712865f},
+    {-0.50000000000000044409f, 0.86602540378443837454f}, {-0.25881904510252062845f, 0.96592582628906831221f},
+    {0.0f, 1.0f}, {0.25881904510252029539f, 0.96592582628906842324f},
+    {0.50000000000000011102f, 0.86602540378443859659f}, {0.70710678118654735069f, 0.70710678118654768376f},
+    {0.86602540378443837454f, 0.50000000000000044409f}, {0.96592582628906809017f, 0.25881904510252157214f}};
+
+
+// exp(-+2 pi i j / R) for R dividing 32 or 24.
 template <int R>
 __device__ __forceinline__ float2 wconst(int j, bool inv) {
+  static_assert(32 % R == 0 || 24 % R == 0, "constant twiddles for R dividing 32 or 24");
   float2 w;
   if constexpr (R == 16) {
     w = {kW16[j][0], kW16[j][1]};
+  } else if constexpr (32 % R == 0) {
+    w = {kW32[j * (32 / R)][0], kW32[j * (32 / R)][1]};
   } else {
-    w = {kW32[j][0], kW32[j][1]};
+    w = {kW24[j * (24 / R)][0], kW24[j * (24 / R)][1]};
   }
   if (inv) w.y = -w.y;
   return w;
@@ -68,6 +87,8 @@ template <>
 __device__ __forceinline__ void dft<4>(float2* v, bool inv) { bfly<4>(v, inv); }
 template <>
 __device__ __forceinline__ void dft<8>(float2* v, bool inv) { bfly<8>(v, inv); }
+template <>
+__device__ __forceinline__ void dft<3>(float2* v, bool inv) { bfly<3>(v, inv); }
 
 // Four-step DFT of A * B values in registers, natural order in and out.
 template <int A, int B>
@@ -98,6 +119,12 @@ template <>
 __device__ __forceinline__ void dft<16>(float2* v, bool inv) { dft_regs<4, 4>(v, inv); }
 template <>
 __device__ __forceinline__ void dft<32>(float2* v, bool inv) { dft_regs<4, 8>(v, inv); }
+template <>
+__device__ __forceinline__ void dft<6>(float2* v, bool inv) { dft_regs<2, 3>(v, inv); }
+template <>
+__device__ __forceinline__ void dft<12>(float2* v, bool inv) { dft_regs<4, 3>(v, inv); }
+template <>
+__device__ __forceinline__ void dft<24>(float2* v, bool inv) { dft_regs<8, 3>(v, inv); }
 
 // LDS index of Y[line][n1][k2] during the transpose. LINE_FAST blocks (threads that walk
 // across lines) keep lines contiguous; the others pad each n1 row so that threads that walk

@@ -10,7 +10,8 @@
 //   store1(f) / store2(f)  f(l, m, v) receives element m after the first / second transform
 // Lines of a block that lie beyond `lines` are neither read nor written.
 //
-// RegLines: lengths N = N1 N2 (32 .. 512) as register-resident four-step transforms; the first
+// RegLines: lengths N = N1 N2 of the form 2^a 3^b (32 .. 512) as register-resident four-step
+// transforms; the first
 // transform leaves thread j with bins N2 k1 + j, which is the input layout of the transform with
 // the factors swapped, so the second transform needs no reordering. BLOCK threads hold
 // BLOCK / max(N1, N2) lines: one-wave blocks suit contiguous rows (many small blocks in flight,

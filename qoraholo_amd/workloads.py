@@ -120,12 +120,12 @@ def bead(n: int = 128, angles: int = 49, seed: int = 0) -> Workload:
                     shapes)
 
 
-def random_cell(n: int, seed: int) -> list:
+def random_cell(n: int, seed: int, pixel: float = 2 * HOLO_PIXEL) -> list:
     """3-8 spheres with n ~ U(1.345, 1.395), each either inside an earlier one or disjoint from
     all of them (the painter's-order rule of the synthesiser), by seeded rejection sampling in
-    a field of view of n voxels of 0.11 um."""
+    a field of view of n voxels of ``pixel`` um (0.11 um by default)."""
     rng = np.random.default_rng(seed)
-    half = 0.4 * n * 2 * HOLO_PIXEL
+    half = 0.4 * n * pixel
     count = int(rng.integers(3, 9))
     placed = []
     while len(placed) < count:
@@ -162,3 +162,37 @@ def mixed_cell(n: int, angles: int, seed: int = 0) -> Workload:
     radius = float(math.ceil(NA / WAVELENGTH / df) + 1)
     return Workload(f"mixed_cell_{n}", holo, bg, (float(c.iy), float(c.ix)), radius, n, geom,
                     holo_geom, est, illum, shapes)
+
+
+BPM_PIXEL = 0.2
+
+
+@dataclass
+class BpmRequest:
+    name: str
+    fields: np.ndarray
+    geometry: Geometry
+    illuminations: np.ndarray
+    truth: np.ndarray
+    shapes: list = field(default_factory=list)
+
+
+def bpm_request(n: int, angles: int, seed: int = 0, noise: float = 0.01) -> BpmRequest:
+    """An inversion request of the served BPM configuration: a ``random_cell`` of side n at
+    0.2 um voxels (n^2 x n), ``angles`` illuminations at NA 0.9, field ratios from the device
+    BPM forward of the rendered cell plus complex Gaussian noise of ``noise`` per component
+    (seeded). The data come from the discretisation that is inverted; the request is a timing
+    and parity workload, not a validation of the model."""
+    from .bpm import bpm_forward
+    from .phantom import render_phantom
+
+    geom = Geometry(WAVELENGTH, N_MEDIUM, NA, BPM_PIXEL)
+    illum = circular_scan(angles, NA_ILL, geom)
+    shapes = random_cell(n, seed, BPM_PIXEL)
+    truth = render_phantom(shapes, geom, n, n)
+    fields = bpm_forward(truth, illum, geom)
+    rng = np.random.default_rng(seed + 100)
+    fields = fields + (noise * (rng.standard_normal(fields.shape, np.float32)
+                                + 1j * rng.standard_normal(fields.shape, np.float32)))
+    return BpmRequest(f"bpm_cell_{n}_{angles}", fields.astype(np.complex64), geom, illum, truth,
+                      shapes)

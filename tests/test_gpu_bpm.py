@@ -33,7 +33,7 @@ def case(qh, nz, n, count=7, na=0.9):
 
 
 @pytest.mark.parametrize("nz,n", [(32, 48), (16, 32), (24, 64), (12, 128), (6, 256), (4, 512),
-                                  (10, 60), (8, 192)])
+                                  (10, 60), (8, 192), (12, 96), (4, 384), (6, 80)])
 def test_forward_matches_oracle(device, qh, oracle, nz, n):
     """Scattered part y - 1 against the FP64 oracle: as close as the float oracle (the
     reference's float algorithm) is, within a factor 2. At large n with a small object the
@@ -227,3 +227,24 @@ def test_inversion_of_exact_bead_on_device(device, qh, oracle):
     rmse = float(np.sqrt(np.mean((ri.astype(np.float64) - truth) ** 2)))
     assert rmse == pytest.approx(3.21e-3, abs=5e-5)
     assert float(ri[inner].mean()) == pytest.approx(1.4019, abs=1e-4)
+
+
+def test_served_request_at_192_matches_oracle(device, qh, oracle):
+    """One inversion request of the served configuration at 192^2 x 192 with 49 illuminations
+    (a random cell at 0.2 um): Rytov + GP 20 start on the device, then 2 FISTA iterations with
+    the TV prox (weight 1e-2), against the oracle from the same start and data."""
+    from qoraholo_amd import workloads
+
+    req = workloads.bpm_request(192, 49, seed=0)
+    g, il, f = req.geometry, req.illuminations, req.fields
+    assert (g.wavelength, g.n_medium, g.na, g.pixel) == GEOM
+    print("bpm served request 192^3 x 49", flush=True)
+    start = qh.reconstruct(qh.rytov(f), il, g, iterations=20)
+    got, gl = qh.bpm_reconstruct(f, il, g, start, iterations=2, tv_weight=1e-2,
+                                 return_loss=True)
+    want, wl = oracle.bpm_reconstruct(f, il, GEOM, start, iterations=2, tv_weight=1e-2)
+    nm = np.float32(GEOM[1])
+    assert rel_l2(got - nm, want - nm) < 1e-4
+    assert np.allclose(gl, wl, rtol=1e-4)
+    assert gl[-1] < gl[0]
+    assert float(got.min()) >= nm

@@ -31,7 +31,8 @@ def cell(qh):
 # ---- transforms ----
 
 @pytest.mark.parametrize("shape", [(8, 16, 32), (6, 10, 12), (4, 96, 80), (3, 128, 256), (2, 512, 8),
-                                   (256, 4, 8), (64, 256, 2), (2, 64, 512)])
+                                   (256, 4, 8), (64, 256, 2), (2, 64, 512), (48, 96, 192),
+                                   (2, 768, 384), (384, 2, 48)])
 def test_fft_c2c_matches_numpy(device, shape):
     from qoraholo_amd import _native
 
@@ -45,7 +46,8 @@ def test_fft_c2c_matches_numpy(device, shape):
         assert rel_l2(got, want) < 2e-6, (shape, forward)
 
 
-@pytest.mark.parametrize("shape", [(16, 32, 64), (12, 20, 30), (64, 64, 128), (128, 128, 128)])
+@pytest.mark.parametrize("shape", [(16, 32, 64), (12, 20, 30), (64, 64, 128), (128, 128, 128),
+                                   (48, 96, 192), (96, 48, 384), (4, 384, 768)])
 def test_real_3d_transforms_match_numpy(device, shape):
     from qoraholo_amd import _native
 
@@ -284,5 +286,27 @@ def test_bead_workload_gp20_matches_oracle(device, oracle, qh):
     want, want_filled = oracle.pipeline(w.holograms, w.backgrounds, 71, 71, w.radius, 128,
                                         w.illuminations, (g.wavelength, g.n_medium, g.na, g.pixel),
                                         iterations=20)
+    assert contrast_error(got, want, g.n_medium) < TOL
+    assert p.filled == want_filled
+
+
+@pytest.mark.parametrize("n,solver", [(192, "gp"), (384, "tv")])
+def test_mixed_stream_sizes_match_oracle(device, oracle, qh, n, solver):
+    """The 3 * 2^k sizes of the served mixed stream end to end (holograms 2n -> n^3): 2n-point
+    hologram rows and columns, n-point field and volume lines, n/2-point packed rows; 9
+    illuminations and 2 iterations keep the oracle's share short."""
+    from qoraholo_amd import workloads
+
+    w = workloads.mixed_cell(n, 9, seed=3)
+    g = w.geometry
+    print(f"pipeline {2 * n}^2 -> {n}^3 {solver}", flush=True)
+    kw = dict(iterations=2, solver=solver)
+    p = qh.Pipeline(9, 2 * n, w.center, w.illuminations, g, radius=w.radius, output_size=n, **kw)
+    p.set_background(w.backgrounds)
+    got = p.run(w.holograms)
+    cy, cx = (int(np.sign(c) * np.floor(abs(c) + 0.5)) for c in w.center)  # lround
+    want, want_filled = oracle.pipeline(w.holograms, w.backgrounds, cy, cx, w.radius, n,
+                                        w.illuminations, (g.wavelength, g.n_medium, g.na, g.pixel),
+                                        **kw)
     assert contrast_error(got, want, g.n_medium) < TOL
     assert p.filled == want_filled
