@@ -436,6 +436,8 @@ def main() -> int:
                          "Poisson arrivals at 0.25x-2x the plain capacity) and print its JSON")
     ap.add_argument("--served-bpm", action="store_true",
                     help="only the served BPM inversion requests (192^3 x 49, 384^3 x 81)")
+    ap.add_argument("--served-bpm-size", type=int, choices=(192, 384), default=0,
+                    help="with --served-bpm: only the request of this size")
     ap.add_argument("--fgp-traffic", action="store_true",
                     help="run only 5 FGP steps at 256^3 and a 256 MiB device copy, for the "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE runs that calibrate HBM traffic")
@@ -460,7 +462,10 @@ def main() -> int:
         print(json.dumps(served_mixed(qh)), flush=True)
         return 0
     if args.served_bpm:
-        print(json.dumps(served_bpm(qh)), flush=True)
+        sizes = ((192, 49), (384, 81))
+        if args.served_bpm_size:
+            sizes = tuple(v for v in sizes if v[0] == args.served_bpm_size)
+        print(json.dumps(served_bpm(qh, sizes=sizes)), flush=True)
         return 0
     if args.bpm_only:
         print(json.dumps(bpm_leg(qh, torch, cpu=not args.no_cpu_baseline, repeats=args.steps)),

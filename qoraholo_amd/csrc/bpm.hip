@@ -324,9 +324,15 @@ void launch_with(Kind k, Args a, unsigned grid, std::size_t lds, hipStream_t s) 
   }
 }
 
+// Rows: one-wave blocks (8 lines of 24 threads, three waves, for 384). Columns: at least 16
+// lines per block, so that each block row is a whole 128-byte segment (blocks of 8 lines would
+// share every cache line with a block that may run on another XCD).
 template <bool ROWS, int N1, int N2>
 void launch_reg(Kind k, Args a, hipStream_t s) {
-  using E = RegLines<N1, N2, !ROWS, ROWS ? 64 : 256>;
+  constexpr int kT = N1 > N2 ? N1 : N2;
+  constexpr int kRows = 64 % kT == 0 ? 64 : 8 * kT;
+  constexpr int kCols = 16 * kT > 256 ? 16 * kT : 256;
+  using E = RegLines<N1, N2, !ROWS, ROWS ? kRows : kCols>;
   a.g.lines_per_block = E::LINES;
   launch_with<ROWS, E>(k, a, E::grid(a.g.lines), 0, s);
 }
@@ -341,7 +347,7 @@ void launch(Kind k, Args a, hipStream_t s) {
     case 128: launch_reg<ROWS, 16, 8>(k, a, s); break;
     case 192: launch_reg<ROWS, 16, 12>(k, a, s); break;
     case 256: launch_reg<ROWS, 16, 16>(k, a, s); break;
-    case 384: launch_reg<ROWS, 32, 12>(k, a, s); break;
+    case 384: launch_reg<ROWS, 16, 24>(k, a, s); break;
     case 512: launch_reg<ROWS, 32, 16>(k, a, s); break;
     default: {
       using E = LdsLines<ROWS>;

@@ -279,22 +279,34 @@ __global__ void __launch_bounds__(kThreads) k_z_roundtrip_restore(ZArgs a) {
   }
 }
 
-// ---- four-step kernels for lengths 2^a 3^b from 32 to 768 (N1 a power of two >= N2) ----
+// ---- four-step kernels for lengths 2^a 3^b from 32 to 768 ----
 
-template <int N1, int N2>
+constexpr int floor_pow2(int v) {
+  int p = 1;
+  while (2 * p <= v) p *= 2;
+  return p;
+}
+
+// T = max(N1, N2) threads per line (a power of two, or 24). Blocks whose threads walk across
+// lines (LINE_FAST) hold at least 16 lines, so that each row of the block is a whole 128-byte
+// segment: with 8 lines, two blocks share every cache line and, scheduled on different XCDs,
+// fetch it twice. Other blocks hold a power of two of lines, about 256 threads.
+template <int N1, int N2, bool LINE_FAST = false>
 struct Shape4 {
   static constexpr int N = N1 * N2;
   static constexpr int T = N1 > N2 ? N1 : N2;
-  static constexpr int LINES = 256 / T;
+  static constexpr int LINES = (LINE_FAST && 256 / T < 16) ? 16 : floor_pow2(256 / T);
+  static constexpr int BLOCK = T * LINES;
+  static_assert(BLOCK % 64 == 0, "whole waves");
 };
 
 template <int N1, int N2, bool LINE_FAST>
-__global__ void __launch_bounds__(256) k_fft4s_lines(LinesArgs a) {
-  using S = Shape4<N1, N2>;
+__global__ void __launch_bounds__((Shape4<N1, N2, LINE_FAST>::BLOCK)) k_fft4s_lines(LinesArgs a) {
+  using S = Shape4<N1, N2, LINE_FAST>;
   __shared__ float2 tw[S::N];
   __shared__ float2 t[t_size<N1, N2, S::LINES, LINE_FAST>()];
   const int tid = static_cast<int>(threadIdx.x);
-  for (int i = tid; i < S::N; i += 256) tw[i] = a.tw[i];
+  for (int i = tid; i < S::N; i += S::BLOCK) tw[i] = a.tw[i];
   const int line = LINE_FAST ? tid % S::LINES : tid / S::T;
   const int j = LINE_FAST ? tid / S::LINES : tid % S::T;
   const long l = static_cast<long>(blockIdx.x) * S::LINES + line;
@@ -324,12 +336,12 @@ __global__ void __launch_bounds__(256) k_fft4s_lines(LinesArgs a) {
 
 // z lines of the half spectrum: forward transform, restore, inverse transform.
 template <int N1, int N2>
-__global__ void __launch_bounds__(256) k_fft4s_zrt(ZArgs a) {
-  using S = Shape4<N1, N2>;
+__global__ void __launch_bounds__((Shape4<N1, N2, true>::BLOCK)) k_fft4s_zrt(ZArgs a) {
+  using S = Shape4<N1, N2, true>;
   __shared__ float2 tw[S::N];
   __shared__ float2 t[t_size<N1, N2, S::LINES, true>()];
   const int tid = static_cast<int>(threadIdx.x);
-  for (int i = tid; i < S::N; i += 256) tw[i] = a.tw[i];
+  for (int i = tid; i < S::N; i += S::BLOCK) tw[i] = a.tw[i];
   const int line = tid % S::LINES;
   const int j = tid / S::LINES;
   const long l = static_cast<long>(blockIdx.x) * S::LINES + line;
@@ -362,15 +374,15 @@ __global__ void __launch_bounds__(256) k_fft4s_zrt(ZArgs a) {
 // Real rows of n = 2 N1 N2 through the packed complex transform, then the split into bins
 // 0 .. n/2 (see k_rfft_rows).
 template <int N1, int N2>
-__global__ void __launch_bounds__(256) k_rfft4s_rows(RowsArgs a) {
+__global__ void __launch_bounds__((Shape4<N1, N2>::BLOCK)) k_rfft4s_rows(RowsArgs a) {
   using S = Shape4<N1, N2>;
   constexpr int M = S::N;
   __shared__ float2 twm[M];
   __shared__ float2 twn[M + 1];
   __shared__ float2 t[t_size<N1, N2, S::LINES, false>()];
   const int tid = static_cast<int>(threadIdx.x);
-  for (int i = tid; i < M; i += 256) twm[i] = a.tw[2 * i];
-  for (int i = tid; i <= M; i += 256) twn[i] = a.tw[i];
+  for (int i = tid; i < M; i += S::BLOCK) twm[i] = a.tw[2 * i];
+  for (int i = tid; i <= M; i += S::BLOCK) twn[i] = a.tw[i];
   const int line = tid / S::T;
   const int j = tid % S::T;
   const long row = static_cast<long>(blockIdx.x) * S::LINES + line;
@@ -416,15 +428,15 @@ __global__ void __launch_bounds__(256) k_rfft4s_rows(RowsArgs a) {
 }
 
 template <int N1, int N2>
-__global__ void __launch_bounds__(256) k_irfft4s_rows(RowsArgs a) {
+__global__ void __launch_bounds__((Shape4<N1, N2>::BLOCK)) k_irfft4s_rows(RowsArgs a) {
   using S = Shape4<N1, N2>;
   constexpr int M = S::N;
   __shared__ float2 twm[M];
   __shared__ float2 twn[M + 1];
   __shared__ float2 t[t_size<N1, N2, S::LINES, false>()];
   const int tid = static_cast<int>(threadIdx.x);
-  for (int i = tid; i < M; i += 256) twm[i] = a.tw[2 * i];
-  for (int i = tid; i <= M; i += 256) twn[i] = a.tw[i];
+  for (int i = tid; i < M; i += S::BLOCK) twm[i] = a.tw[2 * i];
+  for (int i = tid; i <= M; i += S::BLOCK) twn[i] = a.tw[i];
   const int line = tid / S::T;
   const int j = tid % S::T;
   const long row = static_cast<long>(blockIdx.x) * S::LINES + line;
@@ -480,7 +492,7 @@ bool with_four_step(int n, Args&&... args) {
     case 128: F<16, 8>::run(args...); return true;
     case 192: F<16, 12>::run(args...); return true;
     case 256: F<16, 16>::run(args...); return true;
-    case 384: F<32, 12>::run(args...); return true;
+    case 384: F<16, 24>::run(args...); return true;
     case 512: F<32, 16>::run(args...); return true;
     case 768: F<32, 24>::run(args...); return true;
     default: return false;
@@ -490,11 +502,14 @@ bool with_four_step(int n, Args&&... args) {
 template <int N1, int N2>
 struct LaunchLines {
   static void run(const LinesArgs& a, bool contig, hipStream_t s) {
-    const unsigned grid = static_cast<unsigned>((a.g.count + Shape4<N1, N2>::LINES - 1) / Shape4<N1, N2>::LINES);
     if (contig) {
-      hipLaunchKernelGGL((k_fft4s_lines<N1, N2, false>), dim3(grid), dim3(256), 0, s, a);
+      using S = Shape4<N1, N2, false>;
+      const unsigned grid = static_cast<unsigned>((a.g.count + S::LINES - 1) / S::LINES);
+      hipLaunchKernelGGL((k_fft4s_lines<N1, N2, false>), dim3(grid), dim3(S::BLOCK), 0, s, a);
     } else {
-      hipLaunchKernelGGL((k_fft4s_lines<N1, N2, true>), dim3(grid), dim3(256), 0, s, a);
+      using S = Shape4<N1, N2, true>;
+      const unsigned grid = static_cast<unsigned>((a.g.count + S::LINES - 1) / S::LINES);
+      hipLaunchKernelGGL((k_fft4s_lines<N1, N2, true>), dim3(grid), dim3(S::BLOCK), 0, s, a);
     }
   }
 };
@@ -502,8 +517,9 @@ struct LaunchLines {
 template <int N1, int N2>
 struct LaunchZrt {
   static void run(const ZArgs& z, hipStream_t s) {
-    const unsigned grid = static_cast<unsigned>((z.lines + Shape4<N1, N2>::LINES - 1) / Shape4<N1, N2>::LINES);
-    hipLaunchKernelGGL((k_fft4s_zrt<N1, N2>), dim3(grid), dim3(256), 0, s, z);
+    using S = Shape4<N1, N2, true>;
+    const unsigned grid = static_cast<unsigned>((z.lines + S::LINES - 1) / S::LINES);
+    hipLaunchKernelGGL((k_fft4s_zrt<N1, N2>), dim3(grid), dim3(S::BLOCK), 0, s, z);
   }
 };
 
@@ -512,9 +528,9 @@ struct LaunchRows {
   static void run(const RowsArgs& a, bool inverse, hipStream_t s) {
     const unsigned grid = static_cast<unsigned>((a.rows + Shape4<N1, N2>::LINES - 1) / Shape4<N1, N2>::LINES);
     if (inverse) {
-      hipLaunchKernelGGL((k_irfft4s_rows<N1, N2>), dim3(grid), dim3(256), 0, s, a);
+      hipLaunchKernelGGL((k_irfft4s_rows<N1, N2>), dim3(grid), dim3(Shape4<N1, N2>::BLOCK), 0, s, a);
     } else {
-      hipLaunchKernelGGL((k_rfft4s_rows<N1, N2>), dim3(grid), dim3(256), 0, s, a);
+      hipLaunchKernelGGL((k_rfft4s_rows<N1, N2>), dim3(grid), dim3(Shape4<N1, N2>::BLOCK), 0, s, a);
     }
   }
 };

@@ -11,11 +11,12 @@
 // Lines of a block that lie beyond `lines` are neither read nor written.
 //
 // RegLines: lengths N = N1 N2 of the form 2^a 3^b (32 .. 512) as register-resident four-step
-// transforms; the first
+// transforms, max(N1, N2) threads per line; the first
 // transform leaves thread j with bins N2 k1 + j, which is the input layout of the transform with
 // the factors swapped, so the second transform needs no reordering. BLOCK threads hold
 // BLOCK / max(N1, N2) lines: one-wave blocks suit contiguous rows (many small blocks in flight,
-// cheap barriers); columns keep 256 threads so that each load spans whole 128-byte lines.
+// cheap barriers); column blocks hold at least 16 lines so that each load spans whole 128-byte
+// lines.
 // LdsLines: any length that factors into 2, 3 and 5 (up to 2048), lines held in LDS between
 // radix passes, 256 threads.
 #pragma once
@@ -44,7 +45,7 @@ struct RegLines {
   static constexpr int N = N1 * N2;
   static constexpr int T = N1 > N2 ? N1 : N2;
   static constexpr int LINES = BLOCK / T;
-  static_assert(LINES >= 1, "a block must hold at least one line");
+  static_assert(LINES >= 1 && BLOCK % T == 0, "a block holds whole lines");
   static constexpr int kLds =
       ce_max(t_size<N1, N2, LINES, LINE_FAST>(), t_size<N2, N1, LINES, LINE_FAST>());
 
