@@ -347,11 +347,76 @@ __device__ __forceinline__ void lds_get(const float* buf, int row, int nx, int x
   for (int i = 0; i < VX; ++i) v[i] = (xb + i < nx) ? buf[row * nx + xb + i] : 0.0f;
 }
 
-template <int VX, int NW>
+// Row access when nx is a multiple of 4 and VX is too: a lane's VX values are all inside the
+// row or all outside it, so each access is whole float4s (global_load_dwordx4, ds_read_b128)
+// with one predicate per lane instead of one per element.
+template <int VX>
+__device__ __forceinline__ void load_row4(const float* base, const TvArgs& a, int z, int y, int xb,
+                                          float (&out)[VX]) {
+  static_assert(VX % 4 == 0, "whole float4s");
+  if (z >= 0 && z < a.nz && y >= 0 && y < a.ny && xb < a.nx) {
+    const float* p = base + (static_cast<long>(z) * a.ny + y) * a.nx + xb;
+#pragma unroll
+    for (int v = 0; v < VX; v += 4) {
+      const float4 q = *reinterpret_cast<const float4*>(p + v);
+      out[v] = q.x;
+      out[v + 1] = q.y;
+      out[v + 2] = q.z;
+      out[v + 3] = q.w;
+    }
+  } else {
+#pragma unroll
+    for (int v = 0; v < VX; ++v) out[v] = 0.0f;
+  }
+}
+
+template <int VX>
+__device__ __forceinline__ void store_row4(float* base, const TvArgs& a, int z, int y, int xb,
+                                           const float (&in)[VX]) {
+  static_assert(VX % 4 == 0, "whole float4s");
+  if (z < a.nz && y < a.ny && xb < a.nx) {
+    float* p = base + (static_cast<long>(z) * a.ny + y) * a.nx + xb;
+#pragma unroll
+    for (int v = 0; v < VX; v += 4) {
+      *reinterpret_cast<float4*>(p + v) = float4{in[v], in[v + 1], in[v + 2], in[v + 3]};
+    }
+  }
+}
+
+template <int VX>
+__device__ __forceinline__ void lds_put4(float* buf, int row, int nx, int xb, const float (&v)[VX]) {
+  if (xb < nx) {
+#pragma unroll
+    for (int i = 0; i < VX; i += 4) {
+      *reinterpret_cast<float4*>(buf + row * nx + xb + i) = float4{v[i], v[i + 1], v[i + 2], v[i + 3]};
+    }
+  }
+}
+
+template <int VX>
+__device__ __forceinline__ void lds_get4(const float* buf, int row, int nx, int xb, float (&v)[VX]) {
+  if (xb < nx) {
+#pragma unroll
+    for (int i = 0; i < VX; i += 4) {
+      const float4 q = *reinterpret_cast<const float4*>(buf + row * nx + xb + i);
+      v[i] = q.x;
+      v[i + 1] = q.y;
+      v[i + 2] = q.z;
+      v[i + 3] = q.w;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < VX; ++i) v[i] = 0.0f;
+  }
+}
+
+// VEC: nx is a multiple of 4 (and VX of 4): whole-float4 row access.
+template <int VX, int NW, bool VEC>
 __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
   constexpr int TOP = 2;  // halo rows above the output rows (and two below)
   constexpr int T = NW - 4;
-  extern __shared__ float lds[];
+  extern __shared__ float4 lds4[];
+  float* lds = reinterpret_cast<float*>(lds4);
   const int nx = a.nx;
   float* ry = lds;              // y component of r_k of each row
   float* rqy = ry + NW * nx;    // y component of r_{k+1}
@@ -374,19 +439,49 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
   const int wd = w < NW - 1 ? w + 1 : w;  // results only feed other halo values, never a store
   const bool out_row = w >= TOP && w < TOP + T && y < a.ny;
   const int pre = 2;  // planes of warm-up before z0
+  auto load_row = [&](const float* base, const TvArgs& ta, int zz, int yy, int x0, bool v4,
+                      float (&out)[VX]) {
+    if constexpr (VEC) {
+      load_row4<VX>(base, ta, zz, yy, x0, out);
+    } else {
+      qh::load_row<VX>(base, ta, zz, yy, x0, v4, out);
+    }
+  };
+  auto store_row = [&](float* base, const TvArgs& ta, int zz, int yy, int x0, bool v4,
+                       const float (&in)[VX]) {
+    if constexpr (VEC) {
+      store_row4<VX>(base, ta, zz, yy, x0, in);
+    } else {
+      qh::store_row<VX>(base, ta, zz, yy, x0, v4, in);
+    }
+  };
+  auto lds_put = [&](float* buf, int row, int n, int x0, const float (&v)[VX]) {
+    if constexpr (VEC) {
+      lds_put4<VX>(buf, row, n, x0, v);
+    } else {
+      qh::lds_put<VX>(buf, row, n, x0, v);
+    }
+  };
+  auto lds_get = [&](const float* buf, int row, int n, int x0, float (&v)[VX]) {
+    if constexpr (VEC) {
+      lds_get4<VX>(buf, row, n, x0, v);
+    } else {
+      qh::lds_get(buf, row, n, x0, v);
+    }
+  };
   // p_k, p_{k-1} and b of this row on plane f, as loaded.
   auto load_raw = [&](int f, float (&Pk)[3][VX], float (&Pm)[3][VX], float (&B)[VX]) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      load_row<VX>(a.p_k + c * a.vol, bd, f, y, xb, vec, Pk[c]);
+      load_row(a.p_k + c * a.vol, bd, f, y, xb, vec, Pk[c]);
       if (!a.first) {
-        load_row<VX>(a.p_km1 + c * a.vol, bd, f, y, xb, vec, Pm[c]);
+        load_row(a.p_km1 + c * a.vol, bd, f, y, xb, vec, Pm[c]);
       } else {
 #pragma unroll
         for (int v = 0; v < VX; ++v) Pm[c][v] = 0.0f;
       }
     }
-    load_row<VX>(a.b, bd, f, y, xb, vec, B);
+    load_row(a.b, bd, f, y, xb, vec, B);
   };
   // r_k = p_k + m1 (p_k - p_{k-1}), or p_k on the first step.
   auto extrapolate_k = [&](const float (&Pk)[3][VX], const float (&Pm)[3][VX], float (&R)[3][VX]) {
@@ -401,9 +496,9 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       float pm[VX];
-      load_row<VX>(a.p_k + c * a.vol, bd, f, y, xb, vec, Pk[c]);
+      load_row(a.p_k + c * a.vol, bd, f, y, xb, vec, Pk[c]);
       if (!a.first) {
-        load_row<VX>(a.p_km1 + c * a.vol, bd, f, y, xb, vec, pm);
+        load_row(a.p_km1 + c * a.vol, bd, f, y, xb, vec, pm);
       } else {
 #pragma unroll
         for (int v = 0; v < VX; ++v) pm[v] = 0.0f;
@@ -411,7 +506,7 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
 #pragma unroll
       for (int v = 0; v < VX; ++v) R[c][v] = a.first ? Pk[c][v] : Pk[c][v] + a.m1 * (Pk[c][v] - pm[v]);
     }
-    load_row<VX>(a.b, bd, f, y, xb, vec, B);
+    load_row(a.b, bd, f, y, xb, vec, B);
   };
 
   // Prologue: r_k on plane z0 - pre (z component) and z_k on plane z0 - pre + 1.
@@ -419,9 +514,9 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
   float rz_m2[VX];
   {
     float pk[VX], pm[VX];
-    load_row<VX>(a.p_k, bd, zs - 1, y, xb, vec, pk);
+    load_row(a.p_k, bd, zs - 1, y, xb, vec, pk);
     if (!a.first) {
-      load_row<VX>(a.p_km1, bd, zs - 1, y, xb, vec, pm);
+      load_row(a.p_km1, bd, zs - 1, y, xb, vec, pm);
     } else {
 #pragma unroll
       for (int v = 0; v < VX; ++v) pm[v] = 0.0f;
@@ -431,14 +526,14 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
   }
   float rk1[3][VX], pk1[3][VX], b1[VX], zk1[VX];
   load_step1(zs, rk1, pk1, b1);
-  lds_put<VX>(ry, w, nx, xb, rk1[1]);
+  lds_put(ry, w, nx, xb, rk1[1]);
   __syncthreads();
   {
     float up[VX];
-    lds_get<VX>(ry, wu, nx, xb, up);
+    lds_get(ry, wu, nx, xb, up);
     pair_primal<VX>(a, zs, y, xb, rk1, rz_m2, up, b1, zk1);
   }
-  lds_put<VX>(zk, w, nx, xb, zk1);
+  lds_put(zk, w, nx, xb, zk1);
   float rq0[3][VX], zq0[VX], zdn_k[VX], zdn_q[VX];
 #pragma unroll
   for (int v = 0; v < VX; ++v) {
@@ -448,7 +543,7 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
     for (int c = 0; c < 3; ++c) rq0[c][v] = 0.0f;
   }
   __syncthreads();
-  lds_get<VX>(zk, wd, nx, xb, zdn_k);  // z_k of the row below on plane zs
+  lds_get(zk, wd, nx, xb, zdn_k);  // z_k of the row below on plane zs
 
   // The loads of the next plane are issued before the current plane's last dual step, so that
   // their latency overlaps that work instead of stalling the block at the next barrier; r_k is
@@ -459,12 +554,12 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
     const int f = t + 2;
     float rk2[3][VX];
     extrapolate_k(pk2, pm2, rk2);
-    lds_put<VX>(ry, w, nx, xb, rk2[1]);
+    lds_put(ry, w, nx, xb, rk2[1]);
     __syncthreads();  // (1) r_k of plane f visible
     float zk2[VX];
     {
       float up[VX];
-      lds_get<VX>(ry, wu, nx, xb, up);
+      lds_get(ry, wu, nx, xb, up);
       pair_primal<VX>(a, f, y, xb, rk2, rk1[0], up, b2, zk2);
     }
     // p_{k+1} and r_{k+1} on plane t + 1.
@@ -472,24 +567,24 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
     pair_dual<VX>(a, t + 1, y, xb, zk1, zk2, zdn_k, rk1, pq1);
     if (a.p_k1 && out_row && t + 1 >= z0 && t + 1 < z1) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) store_row<VX>(a.p_k1 + c * a.vol, bd, t + 1, y, xb, vec, pq1[c]);
+      for (int c = 0; c < 3; ++c) store_row(a.p_k1 + c * a.vol, bd, t + 1, y, xb, vec, pq1[c]);
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
 #pragma unroll
       for (int v = 0; v < VX; ++v) rq1[c][v] = pq1[c][v] + a.m2 * (pq1[c][v] - pk1[c][v]);
     }
-    lds_put<VX>(zk, w, nx, xb, zk2);
-    lds_put<VX>(rqy, w, nx, xb, rq1[1]);
+    lds_put(zk, w, nx, xb, zk2);
+    lds_put(rqy, w, nx, xb, rq1[1]);
     __syncthreads();  // (2) z_k of plane f and r_{k+1} of plane t + 1 visible
-    lds_get<VX>(zk, wd, nx, xb, zdn_k);  // for the next plane's dual step
+    lds_get(zk, wd, nx, xb, zdn_k);  // for the next plane's dual step
     float zq1[VX];
     {
       float up[VX];
-      lds_get<VX>(rqy, wu, nx, xb, up);
+      lds_get(rqy, wu, nx, xb, up);
       pair_primal<VX>(a, t + 1, y, xb, rq1, rq0[0], up, b1, zq1);
     }
-    lds_put<VX>(zq, w, nx, xb, zq1);
+    lds_put(zq, w, nx, xb, zq1);
     __syncthreads();  // (3) z_{k+1} of plane t + 1 visible
     float pk3[3][VX], pm3[3][VX], b3[VX];
     load_raw(f + 1, pk3, pm3, b3);
@@ -499,10 +594,10 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
       pair_dual<VX>(a, t, y, xb, zq0, zq1, zdn_q, rq0, p2);
       if (out_row) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) store_row<VX>(a.p_k2 + c * a.vol, bd, t, y, xb, vec, p2[c]);
+        for (int c = 0; c < 3; ++c) store_row(a.p_k2 + c * a.vol, bd, t, y, xb, vec, p2[c]);
       }
     }
-    lds_get<VX>(zq, wd, nx, xb, zdn_q);  // z_{k+1} of the row below on plane t + 1
+    lds_get(zq, wd, nx, xb, zdn_q);  // z_{k+1} of the row below on plane t + 1
 #pragma unroll
     for (int v = 0; v < VX; ++v) {
       b1[v] = b2[v];
@@ -658,9 +753,15 @@ void fgp_pair(const float* b, const float* p_k, const float* p_km1, float* p_k1,
   const dim3 block(static_cast<unsigned>(nw * 64));
   const std::size_t lds = 4 * static_cast<std::size_t>(nw) * nx * sizeof(float);
   switch (lanes_x(nx)) {
-    case 1: hipLaunchKernelGGL((k_fgp_pair<1, 16>), grid, block, lds, s, a, chunks); break;
-    case 2: hipLaunchKernelGGL((k_fgp_pair<2, 16>), grid, block, lds, s, a, chunks); break;
-    default: hipLaunchKernelGGL((k_fgp_pair<4, 12>), grid, block, lds, s, a, chunks); break;
+    case 1: hipLaunchKernelGGL((k_fgp_pair<1, 16, false>), grid, block, lds, s, a, chunks); break;
+    case 2: hipLaunchKernelGGL((k_fgp_pair<2, 16, false>), grid, block, lds, s, a, chunks); break;
+    default:
+      if (nx % 4 == 0) {
+        hipLaunchKernelGGL((k_fgp_pair<4, 12, true>), grid, block, lds, s, a, chunks);
+      } else {
+        hipLaunchKernelGGL((k_fgp_pair<4, 12, false>), grid, block, lds, s, a, chunks);
+      }
+      break;
   }
   check_launch("fgp_pair");
 }
