@@ -44,65 +44,73 @@ __device__ __forceinline__ int xcd_tile() {
   return (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
 }
 
-template <int VX>
+// Row access. VEC: nx is a multiple of VX and VX of 2, so a lane's VX values are all inside the
+// row or all outside it and every access is whole float4s (float2s when VX is not a multiple of
+// 4) under one predicate per lane; otherwise element by element.
+template <int VX, bool VEC>
 __device__ __forceinline__ void load_row(const float* base, const TvArgs& a, int z, int y, int xb,
-                                         bool vec, float (&out)[VX]) {
+                                         float (&out)[VX]) {
+  const bool inside = z >= 0 && z < a.nz && y >= 0 && y < a.ny;
+  if constexpr (VEC) {
+    static_assert(VX % 2 == 0, "whole float2s");
+    if (inside && xb < a.nx) {
+      const float* p = base + (static_cast<long>(z) * a.ny + y) * a.nx + xb;
+      if constexpr (VX % 4 == 0) {
 #pragma unroll
-  for (int v = 0; v < VX; ++v) out[v] = 0.0f;
-  if (z < 0 || z >= a.nz || y < 0 || y >= a.ny) return;
-  const float* p = base + (static_cast<long>(z) * a.ny + y) * a.nx + xb;
-  if constexpr (VX % 4 == 0) {
-    if (vec && xb + VX <= a.nx) {
+        for (int v = 0; v < VX; v += 4) {
+          const float4 q = *reinterpret_cast<const float4*>(p + v);
+          out[v] = q.x;
+          out[v + 1] = q.y;
+          out[v + 2] = q.z;
+          out[v + 3] = q.w;
+        }
+      } else {
 #pragma unroll
-      for (int v = 0; v < VX; v += 4) {
-        const float4 q = *reinterpret_cast<const float4*>(p + v);
-        out[v] = q.x;
-        out[v + 1] = q.y;
-        out[v + 2] = q.z;
-        out[v + 3] = q.w;
+        for (int v = 0; v < VX; v += 2) {
+          const float2 q = *reinterpret_cast<const float2*>(p + v);
+          out[v] = q.x;
+          out[v + 1] = q.y;
+        }
       }
-      return;
-    }
-  } else if constexpr (VX % 2 == 0) {
-    if (vec && xb + VX <= a.nx) {
+    } else {
 #pragma unroll
-      for (int v = 0; v < VX; v += 2) {
-        const float2 q = *reinterpret_cast<const float2*>(p + v);
-        out[v] = q.x;
-        out[v + 1] = q.y;
-      }
-      return;
+      for (int v = 0; v < VX; ++v) out[v] = 0.0f;
     }
-  }
+  } else {
 #pragma unroll
-  for (int v = 0; v < VX; ++v) {
-    if (xb + v < a.nx) out[v] = p[v];
+    for (int v = 0; v < VX; ++v) out[v] = 0.0f;
+    if (!inside) return;
+    const float* p = base + (static_cast<long>(z) * a.ny + y) * a.nx + xb;
+#pragma unroll
+    for (int v = 0; v < VX; ++v) {
+      if (xb + v < a.nx) out[v] = p[v];
+    }
   }
 }
 
-template <int VX>
+template <int VX, bool VEC>
 __device__ __forceinline__ void store_row(float* base, const TvArgs& a, int z, int y, int xb,
-                                          bool vec, const float (&in)[VX]) {
+                                          const float (&in)[VX]) {
   if (z >= a.nz || y >= a.ny) return;
   float* p = base + (static_cast<long>(z) * a.ny + y) * a.nx + xb;
-  if constexpr (VX % 4 == 0) {
-    if (vec && xb + VX <= a.nx) {
+  if constexpr (VEC) {
+    static_assert(VX % 2 == 0, "whole float2s");
+    if (xb < a.nx) {
+      if constexpr (VX % 4 == 0) {
 #pragma unroll
-      for (int v = 0; v < VX; v += 4) {
-        *reinterpret_cast<float4*>(p + v) = float4{in[v], in[v + 1], in[v + 2], in[v + 3]};
+        for (int v = 0; v < VX; v += 4) {
+          *reinterpret_cast<float4*>(p + v) = float4{in[v], in[v + 1], in[v + 2], in[v + 3]};
+        }
+      } else {
+#pragma unroll
+        for (int v = 0; v < VX; v += 2) *reinterpret_cast<float2*>(p + v) = float2{in[v], in[v + 1]};
       }
-      return;
     }
-  } else if constexpr (VX % 2 == 0) {
-    if (vec && xb + VX <= a.nx) {
+  } else {
 #pragma unroll
-      for (int v = 0; v < VX; v += 2) *reinterpret_cast<float2*>(p + v) = float2{in[v], in[v + 1]};
-      return;
+    for (int v = 0; v < VX; ++v) {
+      if (xb + v < a.nx) p[v] = in[v];
     }
-  }
-#pragma unroll
-  for (int v = 0; v < VX; ++v) {
-    if (xb + v < a.nx) p[v] = in[v];
   }
 }
 
@@ -147,8 +155,8 @@ __device__ __forceinline__ void extrapolate(const TvArgs& a, const float (&pk)[V
 // r of plane z on the strip rows and the halo row below (R), its y component on the row above
 // (ry_top), and b on the strip rows and the halo row. P keeps p_k of the strip rows for the
 // dual step of this plane.
-template <int VX, int TY>
-__device__ __forceinline__ void load_plane(const TvArgs& a, int z, int y0, int xb, bool vec,
+template <int VX, int TY, bool VEC>
+__device__ __forceinline__ void load_plane(const TvArgs& a, int z, int y0, int xb,
                                            float (&R)[3][TY + 1][VX], float (&ry_top)[VX],
                                            float (&B)[TY + 1][VX], float (&P)[3][TY][VX]) {
 #pragma unroll
@@ -156,9 +164,9 @@ __device__ __forceinline__ void load_plane(const TvArgs& a, int z, int y0, int x
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       float pk[VX], pm[VX];
-      load_row<VX>(a.p_in + c * a.vol, a, z, y0 + j, xb, vec, pk);
+      load_row<VX, VEC>(a.p_in + c * a.vol, a, z, y0 + j, xb, pk);
       if (!a.first) {
-        load_row<VX>(a.p_prev + c * a.vol, a, z, y0 + j, xb, vec, pm);
+        load_row<VX, VEC>(a.p_prev + c * a.vol, a, z, y0 + j, xb, pm);
       } else {
 #pragma unroll
         for (int v = 0; v < VX; ++v) pm[v] = 0.0f;
@@ -169,12 +177,12 @@ __device__ __forceinline__ void load_plane(const TvArgs& a, int z, int y0, int x
         for (int v = 0; v < VX; ++v) P[c][j][v] = pk[v];
       }
     }
-    load_row<VX>(a.b, a, z, y0 + j, xb, vec, B[j]);
+    load_row<VX, VEC>(a.b, a, z, y0 + j, xb, B[j]);
   }
   float pk[VX], pm[VX];
-  load_row<VX>(a.p_in + a.vol, a, z, y0 - 1, xb, vec, pk);
+  load_row<VX, VEC>(a.p_in + a.vol, a, z, y0 - 1, xb, pk);
   if (!a.first) {
-    load_row<VX>(a.p_prev + a.vol, a, z, y0 - 1, xb, vec, pm);
+    load_row<VX, VEC>(a.p_prev + a.vol, a, z, y0 - 1, xb, pm);
   } else {
 #pragma unroll
     for (int v = 0; v < VX; ++v) pm[v] = 0.0f;
@@ -182,14 +190,13 @@ __device__ __forceinline__ void load_plane(const TvArgs& a, int z, int y0, int x
   extrapolate<VX>(a, pk, pm, ry_top);
 }
 
-template <int VX, int TY>
+template <int VX, int TY, bool VEC>
 __global__ void __launch_bounds__(64) k_fgp_step(TvArgs a, int chunks) {
   const int t = xcd_tile();
   const int y0 = (t / chunks) * TY;
   const int z0 = (t % chunks) * kZChunk;
   const int z1 = min(z0 + kZChunk, a.nz);
   const int xb = static_cast<int>(threadIdx.x) * VX;
-  const bool vec = (a.nx % 4) == 0;
 
   float r[3][TY + 1][VX];  // r at plane z, rows y0..y0+TY
   float pc[3][TY][VX];     // p_k at plane z, strip rows
@@ -200,9 +207,9 @@ __global__ void __launch_bounds__(64) k_fgp_step(TvArgs a, int chunks) {
 #pragma unroll
     for (int j = 0; j <= TY; ++j) {
       float pk[VX], pm[VX];
-      load_row<VX>(a.p_in, a, z0 - 1, y0 + j, xb, vec, pk);
+      load_row<VX, VEC>(a.p_in, a, z0 - 1, y0 + j, xb, pk);
       if (!a.first) {
-        load_row<VX>(a.p_prev, a, z0 - 1, y0 + j, xb, vec, pm);
+        load_row<VX, VEC>(a.p_prev, a, z0 - 1, y0 + j, xb, pm);
       } else {
 #pragma unroll
         for (int v = 0; v < VX; ++v) pm[v] = 0.0f;
@@ -210,12 +217,12 @@ __global__ void __launch_bounds__(64) k_fgp_step(TvArgs a, int chunks) {
       extrapolate<VX>(a, pk, pm, rz_below[j]);
     }
     float ry_top[VX], bb[TY + 1][VX];
-    load_plane<VX, TY>(a, z0, y0, xb, vec, r, ry_top, bb, pc);
+    load_plane<VX, TY, VEC>(a, z0, y0, xb, r, ry_top, bb, pc);
     primal_plane<VX, TY>(a, z0, y0, xb, r, ry_top, rz_below, bb, zc);
   }
   for (int z = z0; z < z1; ++z) {
     float rn[3][TY + 1][VX], ryn[VX], bn[TY + 1][VX], pn[3][TY][VX], zn[TY + 1][VX];
-    load_plane<VX, TY>(a, z + 1, y0, xb, vec, rn, ryn, bn, pn);
+    load_plane<VX, TY, VEC>(a, z + 1, y0, xb, rn, ryn, bn, pn);
     primal_plane<VX, TY>(a, z + 1, y0, xb, rn, ryn, r[0], bn, zn);
 #pragma unroll
     for (int j = 0; j < TY; ++j) {
@@ -237,7 +244,7 @@ __global__ void __launch_bounds__(64) k_fgp_step(TvArgs a, int chunks) {
         for (int cc = 0; cc < 3; ++cc) po[cc][v] = p3[cc];
       }
 #pragma unroll
-      for (int cc = 0; cc < 3; ++cc) store_row<VX>(a.p_out + cc * a.vol, a, z, y, xb, vec, po[cc]);
+      for (int cc = 0; cc < 3; ++cc) store_row<VX, VEC>(a.p_out + cc * a.vol, a, z, y, xb, po[cc]);
     }
 #pragma unroll
     for (int j = 0; j <= TY; ++j) {
@@ -333,84 +340,47 @@ __device__ __forceinline__ void pair_dual(const PairArgs& a, int zp, int y, int 
   }
 }
 
-template <int VX>
+// Rows in LDS (row r of a buffer at r * nx), with the access rules of load_row.
+template <int VX, bool VEC>
 __device__ __forceinline__ void lds_put(float* buf, int row, int nx, int xb, const float (&v)[VX]) {
+  if constexpr (VEC && VX % 4 == 0) {
+    if (xb < nx) {
 #pragma unroll
-  for (int i = 0; i < VX; ++i) {
-    if (xb + i < nx) buf[row * nx + xb + i] = v[i];
+      for (int i = 0; i < VX; i += 4) {
+        *reinterpret_cast<float4*>(buf + row * nx + xb + i) = float4{v[i], v[i + 1], v[i + 2], v[i + 3]};
+      }
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < VX; ++i) {
+      if (xb + i < nx) buf[row * nx + xb + i] = v[i];
+    }
   }
 }
 
-template <int VX>
+template <int VX, bool VEC>
 __device__ __forceinline__ void lds_get(const float* buf, int row, int nx, int xb, float (&v)[VX]) {
+  if constexpr (VEC && VX % 4 == 0) {
+    if (xb < nx) {
 #pragma unroll
-  for (int i = 0; i < VX; ++i) v[i] = (xb + i < nx) ? buf[row * nx + xb + i] : 0.0f;
-}
-
-// Row access when nx is a multiple of 4 and VX is too: a lane's VX values are all inside the
-// row or all outside it, so each access is whole float4s (global_load_dwordx4, ds_read_b128)
-// with one predicate per lane instead of one per element.
-template <int VX>
-__device__ __forceinline__ void load_row4(const float* base, const TvArgs& a, int z, int y, int xb,
-                                          float (&out)[VX]) {
-  static_assert(VX % 4 == 0, "whole float4s");
-  if (z >= 0 && z < a.nz && y >= 0 && y < a.ny && xb < a.nx) {
-    const float* p = base + (static_cast<long>(z) * a.ny + y) * a.nx + xb;
+      for (int i = 0; i < VX; i += 4) {
+        const float4 q = *reinterpret_cast<const float4*>(buf + row * nx + xb + i);
+        v[i] = q.x;
+        v[i + 1] = q.y;
+        v[i + 2] = q.z;
+        v[i + 3] = q.w;
+      }
+    } else {
 #pragma unroll
-    for (int v = 0; v < VX; v += 4) {
-      const float4 q = *reinterpret_cast<const float4*>(p + v);
-      out[v] = q.x;
-      out[v + 1] = q.y;
-      out[v + 2] = q.z;
-      out[v + 3] = q.w;
+      for (int i = 0; i < VX; ++i) v[i] = 0.0f;
     }
   } else {
 #pragma unroll
-    for (int v = 0; v < VX; ++v) out[v] = 0.0f;
+    for (int i = 0; i < VX; ++i) v[i] = (xb + i < nx) ? buf[row * nx + xb + i] : 0.0f;
   }
 }
 
-template <int VX>
-__device__ __forceinline__ void store_row4(float* base, const TvArgs& a, int z, int y, int xb,
-                                           const float (&in)[VX]) {
-  static_assert(VX % 4 == 0, "whole float4s");
-  if (z < a.nz && y < a.ny && xb < a.nx) {
-    float* p = base + (static_cast<long>(z) * a.ny + y) * a.nx + xb;
-#pragma unroll
-    for (int v = 0; v < VX; v += 4) {
-      *reinterpret_cast<float4*>(p + v) = float4{in[v], in[v + 1], in[v + 2], in[v + 3]};
-    }
-  }
-}
-
-template <int VX>
-__device__ __forceinline__ void lds_put4(float* buf, int row, int nx, int xb, const float (&v)[VX]) {
-  if (xb < nx) {
-#pragma unroll
-    for (int i = 0; i < VX; i += 4) {
-      *reinterpret_cast<float4*>(buf + row * nx + xb + i) = float4{v[i], v[i + 1], v[i + 2], v[i + 3]};
-    }
-  }
-}
-
-template <int VX>
-__device__ __forceinline__ void lds_get4(const float* buf, int row, int nx, int xb, float (&v)[VX]) {
-  if (xb < nx) {
-#pragma unroll
-    for (int i = 0; i < VX; i += 4) {
-      const float4 q = *reinterpret_cast<const float4*>(buf + row * nx + xb + i);
-      v[i] = q.x;
-      v[i + 1] = q.y;
-      v[i + 2] = q.z;
-      v[i + 3] = q.w;
-    }
-  } else {
-#pragma unroll
-    for (int i = 0; i < VX; ++i) v[i] = 0.0f;
-  }
-}
-
-// VEC: nx is a multiple of 4 (and VX of 4): whole-float4 row access.
+// VEC: see load_row.
 template <int VX, int NW, bool VEC>
 __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
   constexpr int TOP = 2;  // halo rows above the output rows (and two below)
@@ -434,54 +404,23 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
   const int z1 = min(z0 + kPairChunk, a.nz);
   const int y = y0 - TOP + w;
   const int xb = lane * VX;
-  const bool vec = (nx % 4) == 0;
   const int wu = w > 0 ? w - 1 : 0;       // the first and last waves read their own row: their
   const int wd = w < NW - 1 ? w + 1 : w;  // results only feed other halo values, never a store
   const bool out_row = w >= TOP && w < TOP + T && y < a.ny;
   const int pre = 2;  // planes of warm-up before z0
-  auto load_row = [&](const float* base, const TvArgs& ta, int zz, int yy, int x0, bool v4,
-                      float (&out)[VX]) {
-    if constexpr (VEC) {
-      load_row4<VX>(base, ta, zz, yy, x0, out);
-    } else {
-      qh::load_row<VX>(base, ta, zz, yy, x0, v4, out);
-    }
-  };
-  auto store_row = [&](float* base, const TvArgs& ta, int zz, int yy, int x0, bool v4,
-                       const float (&in)[VX]) {
-    if constexpr (VEC) {
-      store_row4<VX>(base, ta, zz, yy, x0, in);
-    } else {
-      qh::store_row<VX>(base, ta, zz, yy, x0, v4, in);
-    }
-  };
-  auto lds_put = [&](float* buf, int row, int n, int x0, const float (&v)[VX]) {
-    if constexpr (VEC) {
-      lds_put4<VX>(buf, row, n, x0, v);
-    } else {
-      qh::lds_put<VX>(buf, row, n, x0, v);
-    }
-  };
-  auto lds_get = [&](const float* buf, int row, int n, int x0, float (&v)[VX]) {
-    if constexpr (VEC) {
-      lds_get4<VX>(buf, row, n, x0, v);
-    } else {
-      qh::lds_get(buf, row, n, x0, v);
-    }
-  };
   // p_k, p_{k-1} and b of this row on plane f, as loaded.
   auto load_raw = [&](int f, float (&Pk)[3][VX], float (&Pm)[3][VX], float (&B)[VX]) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      load_row(a.p_k + c * a.vol, bd, f, y, xb, vec, Pk[c]);
+      load_row<VX, VEC>(a.p_k + c * a.vol, bd, f, y, xb, Pk[c]);
       if (!a.first) {
-        load_row(a.p_km1 + c * a.vol, bd, f, y, xb, vec, Pm[c]);
+        load_row<VX, VEC>(a.p_km1 + c * a.vol, bd, f, y, xb, Pm[c]);
       } else {
 #pragma unroll
         for (int v = 0; v < VX; ++v) Pm[c][v] = 0.0f;
       }
     }
-    load_row(a.b, bd, f, y, xb, vec, B);
+    load_row<VX, VEC>(a.b, bd, f, y, xb, B);
   };
   // r_k = p_k + m1 (p_k - p_{k-1}), or p_k on the first step.
   auto extrapolate_k = [&](const float (&Pk)[3][VX], const float (&Pm)[3][VX], float (&R)[3][VX]) {
@@ -496,9 +435,9 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       float pm[VX];
-      load_row(a.p_k + c * a.vol, bd, f, y, xb, vec, Pk[c]);
+      load_row<VX, VEC>(a.p_k + c * a.vol, bd, f, y, xb, Pk[c]);
       if (!a.first) {
-        load_row(a.p_km1 + c * a.vol, bd, f, y, xb, vec, pm);
+        load_row<VX, VEC>(a.p_km1 + c * a.vol, bd, f, y, xb, pm);
       } else {
 #pragma unroll
         for (int v = 0; v < VX; ++v) pm[v] = 0.0f;
@@ -506,7 +445,7 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
 #pragma unroll
       for (int v = 0; v < VX; ++v) R[c][v] = a.first ? Pk[c][v] : Pk[c][v] + a.m1 * (Pk[c][v] - pm[v]);
     }
-    load_row(a.b, bd, f, y, xb, vec, B);
+    load_row<VX, VEC>(a.b, bd, f, y, xb, B);
   };
 
   // Prologue: r_k on plane z0 - pre (z component) and z_k on plane z0 - pre + 1.
@@ -514,9 +453,9 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
   float rz_m2[VX];
   {
     float pk[VX], pm[VX];
-    load_row(a.p_k, bd, zs - 1, y, xb, vec, pk);
+    load_row<VX, VEC>(a.p_k, bd, zs - 1, y, xb, pk);
     if (!a.first) {
-      load_row(a.p_km1, bd, zs - 1, y, xb, vec, pm);
+      load_row<VX, VEC>(a.p_km1, bd, zs - 1, y, xb, pm);
     } else {
 #pragma unroll
       for (int v = 0; v < VX; ++v) pm[v] = 0.0f;
@@ -526,14 +465,14 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
   }
   float rk1[3][VX], pk1[3][VX], b1[VX], zk1[VX];
   load_step1(zs, rk1, pk1, b1);
-  lds_put(ry, w, nx, xb, rk1[1]);
+  lds_put<VX, VEC>(ry, w, nx, xb, rk1[1]);
   __syncthreads();
   {
     float up[VX];
-    lds_get(ry, wu, nx, xb, up);
+    lds_get<VX, VEC>(ry, wu, nx, xb, up);
     pair_primal<VX>(a, zs, y, xb, rk1, rz_m2, up, b1, zk1);
   }
-  lds_put(zk, w, nx, xb, zk1);
+  lds_put<VX, VEC>(zk, w, nx, xb, zk1);
   float rq0[3][VX], zq0[VX], zdn_k[VX], zdn_q[VX];
 #pragma unroll
   for (int v = 0; v < VX; ++v) {
@@ -543,7 +482,7 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
     for (int c = 0; c < 3; ++c) rq0[c][v] = 0.0f;
   }
   __syncthreads();
-  lds_get(zk, wd, nx, xb, zdn_k);  // z_k of the row below on plane zs
+  lds_get<VX, VEC>(zk, wd, nx, xb, zdn_k);  // z_k of the row below on plane zs
 
   // The loads of the next plane are issued before the current plane's last dual step, so that
   // their latency overlaps that work instead of stalling the block at the next barrier; r_k is
@@ -554,12 +493,12 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
     const int f = t + 2;
     float rk2[3][VX];
     extrapolate_k(pk2, pm2, rk2);
-    lds_put(ry, w, nx, xb, rk2[1]);
+    lds_put<VX, VEC>(ry, w, nx, xb, rk2[1]);
     __syncthreads();  // (1) r_k of plane f visible
     float zk2[VX];
     {
       float up[VX];
-      lds_get(ry, wu, nx, xb, up);
+      lds_get<VX, VEC>(ry, wu, nx, xb, up);
       pair_primal<VX>(a, f, y, xb, rk2, rk1[0], up, b2, zk2);
     }
     // p_{k+1} and r_{k+1} on plane t + 1.
@@ -567,24 +506,24 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
     pair_dual<VX>(a, t + 1, y, xb, zk1, zk2, zdn_k, rk1, pq1);
     if (a.p_k1 && out_row && t + 1 >= z0 && t + 1 < z1) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) store_row(a.p_k1 + c * a.vol, bd, t + 1, y, xb, vec, pq1[c]);
+      for (int c = 0; c < 3; ++c) store_row<VX, VEC>(a.p_k1 + c * a.vol, bd, t + 1, y, xb, pq1[c]);
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
 #pragma unroll
       for (int v = 0; v < VX; ++v) rq1[c][v] = pq1[c][v] + a.m2 * (pq1[c][v] - pk1[c][v]);
     }
-    lds_put(zk, w, nx, xb, zk2);
-    lds_put(rqy, w, nx, xb, rq1[1]);
+    lds_put<VX, VEC>(zk, w, nx, xb, zk2);
+    lds_put<VX, VEC>(rqy, w, nx, xb, rq1[1]);
     __syncthreads();  // (2) z_k of plane f and r_{k+1} of plane t + 1 visible
-    lds_get(zk, wd, nx, xb, zdn_k);  // for the next plane's dual step
+    lds_get<VX, VEC>(zk, wd, nx, xb, zdn_k);  // for the next plane's dual step
     float zq1[VX];
     {
       float up[VX];
-      lds_get(rqy, wu, nx, xb, up);
+      lds_get<VX, VEC>(rqy, wu, nx, xb, up);
       pair_primal<VX>(a, t + 1, y, xb, rq1, rq0[0], up, b1, zq1);
     }
-    lds_put(zq, w, nx, xb, zq1);
+    lds_put<VX, VEC>(zq, w, nx, xb, zq1);
     __syncthreads();  // (3) z_{k+1} of plane t + 1 visible
     float pk3[3][VX], pm3[3][VX], b3[VX];
     load_raw(f + 1, pk3, pm3, b3);
@@ -594,10 +533,10 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
       pair_dual<VX>(a, t, y, xb, zq0, zq1, zdn_q, rq0, p2);
       if (out_row) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) store_row(a.p_k2 + c * a.vol, bd, t, y, xb, vec, p2[c]);
+        for (int c = 0; c < 3; ++c) store_row<VX, VEC>(a.p_k2 + c * a.vol, bd, t, y, xb, p2[c]);
       }
     }
-    lds_get(zq, wd, nx, xb, zdn_q);  // z_{k+1} of the row below on plane t + 1
+    lds_get<VX, VEC>(zq, wd, nx, xb, zdn_q);  // z_{k+1} of the row below on plane t + 1
 #pragma unroll
     for (int v = 0; v < VX; ++v) {
       b1[v] = b2[v];
@@ -617,7 +556,7 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
 }
 
 // out = max(b - lambda L(p), 0) (optionally its refractive index), same strip layout.
-template <int VX, int TY>
+template <int VX, int TY, bool VEC>
 __global__ void __launch_bounds__(64) k_tv_primal(TvArgs a, int chunks, float* out, int to_index,
                                                   float nm2, float inv_k02) {
   const int t = xcd_tile();
@@ -625,29 +564,28 @@ __global__ void __launch_bounds__(64) k_tv_primal(TvArgs a, int chunks, float* o
   const int z0 = (t % chunks) * kZChunk;
   const int z1 = min(z0 + kZChunk, a.nz);
   const int xb = static_cast<int>(threadIdx.x) * VX;
-  const bool vec = (a.nx % 4) == 0;
   float p[3][TY + 1][VX];
   float pz_below[TY + 1][VX];
   float py_top[VX];
   float bb[TY + 1][VX];
   float x3[TY + 1][VX];
 #pragma unroll
-  for (int j = 0; j < TY; ++j) load_row<VX>(a.p_in, a, z0 - 1, y0 + j, xb, vec, pz_below[j]);
+  for (int j = 0; j < TY; ++j) load_row<VX, VEC>(a.p_in, a, z0 - 1, y0 + j, xb, pz_below[j]);
 #pragma unroll
   for (int v = 0; v < VX; ++v) pz_below[TY][v] = 0.0f;
   for (int z = z0; z < z1; ++z) {
 #pragma unroll
     for (int j = 0; j < TY; ++j) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) load_row<VX>(a.p_in + c * a.vol, a, z, y0 + j, xb, vec, p[c][j]);
-      load_row<VX>(a.b, a, z, y0 + j, xb, vec, bb[j]);
+      for (int c = 0; c < 3; ++c) load_row<VX, VEC>(a.p_in + c * a.vol, a, z, y0 + j, xb, p[c][j]);
+      load_row<VX, VEC>(a.b, a, z, y0 + j, xb, bb[j]);
     }
 #pragma unroll
     for (int v = 0; v < VX; ++v) {
       p[0][TY][v] = p[1][TY][v] = p[2][TY][v] = 0.0f;
       bb[TY][v] = 0.0f;
     }
-    load_row<VX>(a.p_in + a.vol, a, z, y0 - 1, xb, vec, py_top);
+    load_row<VX, VEC>(a.p_in + a.vol, a, z, y0 - 1, xb, py_top);
     primal_plane<VX, TY>(a, z, y0, xb, p, py_top, pz_below, bb, x3);
 #pragma unroll
     for (int j = 0; j < TY; ++j) {
@@ -657,7 +595,7 @@ __global__ void __launch_bounds__(64) k_tv_primal(TvArgs a, int chunks, float* o
         o[v] = to_index ? index_from_potential(x3[j][v], 0.0f, nm2, inv_k02) : x3[j][v];
         pz_below[j][v] = p[0][j][v];
       }
-      store_row<VX>(out, a, z, y0 + j, xb, vec, o);
+      store_row<VX, VEC>(out, a, z, y0 + j, xb, o);
     }
   }
 }
@@ -694,6 +632,37 @@ dim3 strip_grid(int ny, int nz, int* chunks) {
   return dim3(static_cast<unsigned>(((ny + kRows - 1) / kRows) * *chunks));
 }
 
+// Whole-vector row access (see load_row) when the row splits into whole lane chunks.
+template <int VX>
+bool rows_vec(int nx) {
+  return VX % 2 == 0 && nx % VX == 0;
+}
+
+template <int VX>
+void launch_step(const TvArgs& a, int chunks, dim3 grid, hipStream_t s) {
+  if constexpr (VX % 2 == 0) {
+    if (rows_vec<VX>(a.nx)) {
+      hipLaunchKernelGGL((k_fgp_step<VX, kRows, true>), grid, dim3(64), 0, s, a, chunks);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_fgp_step<VX, kRows, false>), grid, dim3(64), 0, s, a, chunks);
+}
+
+template <int VX>
+void launch_primal(const TvArgs& a, int chunks, dim3 grid, float* out, int ti, float nm2,
+                   float inv_k02, hipStream_t s) {
+  if constexpr (VX % 2 == 0) {
+    if (rows_vec<VX>(a.nx)) {
+      hipLaunchKernelGGL((k_tv_primal<VX, kRows, true>), grid, dim3(64), 0, s, a, chunks, out, ti,
+                         nm2, inv_k02);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_tv_primal<VX, kRows, false>), grid, dim3(64), 0, s, a, chunks, out, ti,
+                     nm2, inv_k02);
+}
+
 }  // namespace
 
 void fgp_step(const float* b, const float* p_in, const float* p_prev, float* p_out, int nz,
@@ -704,11 +673,11 @@ void fgp_step(const float* b, const float* p_in, const float* p_prev, float* p_o
   int chunks = 0;
   const dim3 grid = strip_grid(ny, nz, &chunks);
   switch (lanes_x(nx)) {
-    case 1: hipLaunchKernelGGL((k_fgp_step<1, kRows>), grid, dim3(64), 0, s, a, chunks); break;
-    case 2: hipLaunchKernelGGL((k_fgp_step<2, kRows>), grid, dim3(64), 0, s, a, chunks); break;
-    case 4: hipLaunchKernelGGL((k_fgp_step<4, kRows>), grid, dim3(64), 0, s, a, chunks); break;
-    case 6: hipLaunchKernelGGL((k_fgp_step<6, kRows>), grid, dim3(64), 0, s, a, chunks); break;
-    default: hipLaunchKernelGGL((k_fgp_step<8, kRows>), grid, dim3(64), 0, s, a, chunks); break;
+    case 1: launch_step<1>(a, chunks, grid, s); break;
+    case 2: launch_step<2>(a, chunks, grid, s); break;
+    case 4: launch_step<4>(a, chunks, grid, s); break;
+    case 6: launch_step<6>(a, chunks, grid, s); break;
+    default: launch_step<8>(a, chunks, grid, s); break;
   }
   check_launch("fgp_step");
 }
@@ -721,11 +690,11 @@ void tv_primal(const float* b, const float* p, int nz, int ny, int nx, float lam
   const dim3 grid = strip_grid(ny, nz, &chunks);
   const int ti = to_index ? 1 : 0;
   switch (lanes_x(nx)) {
-    case 1: hipLaunchKernelGGL((k_tv_primal<1, kRows>), grid, dim3(64), 0, s, a, chunks, out, ti, nm2, inv_k02); break;
-    case 2: hipLaunchKernelGGL((k_tv_primal<2, kRows>), grid, dim3(64), 0, s, a, chunks, out, ti, nm2, inv_k02); break;
-    case 4: hipLaunchKernelGGL((k_tv_primal<4, kRows>), grid, dim3(64), 0, s, a, chunks, out, ti, nm2, inv_k02); break;
-    case 6: hipLaunchKernelGGL((k_tv_primal<6, kRows>), grid, dim3(64), 0, s, a, chunks, out, ti, nm2, inv_k02); break;
-    default: hipLaunchKernelGGL((k_tv_primal<8, kRows>), grid, dim3(64), 0, s, a, chunks, out, ti, nm2, inv_k02); break;
+    case 1: launch_primal<1>(a, chunks, grid, out, ti, nm2, inv_k02, s); break;
+    case 2: launch_primal<2>(a, chunks, grid, out, ti, nm2, inv_k02, s); break;
+    case 4: launch_primal<4>(a, chunks, grid, out, ti, nm2, inv_k02, s); break;
+    case 6: launch_primal<6>(a, chunks, grid, out, ti, nm2, inv_k02, s); break;
+    default: launch_primal<8>(a, chunks, grid, out, ti, nm2, inv_k02, s); break;
   }
   check_launch("tv_primal");
 }
@@ -756,7 +725,7 @@ void fgp_pair(const float* b, const float* p_k, const float* p_km1, float* p_k1,
     case 1: hipLaunchKernelGGL((k_fgp_pair<1, 16, false>), grid, block, lds, s, a, chunks); break;
     case 2: hipLaunchKernelGGL((k_fgp_pair<2, 16, false>), grid, block, lds, s, a, chunks); break;
     default:
-      if (nx % 4 == 0) {
+      if (rows_vec<4>(nx)) {
         hipLaunchKernelGGL((k_fgp_pair<4, 12, true>), grid, block, lds, s, a, chunks);
       } else {
         hipLaunchKernelGGL((k_fgp_pair<4, 12, false>), grid, block, lds, s, a, chunks);
