@@ -277,7 +277,11 @@ __global__ void __launch_bounds__(64) k_fgp_step(TvArgs a, int chunks) {
 // side make step one available where step two needs it; their results are not stored. Every
 // value is formed with the float operations of k_fgp_step, so a pass equals two single steps
 // bit for bit, while HBM sees 7 volumes read and 6 written per two steps instead of 20.
+// Planes per block: 32, or fewer (down to 8) when that leaves the grid short of one block per
+// CU; each chunk starts two planes early.
 constexpr int kPairChunk = 32;
+constexpr int kPairChunkMin = 8;
+constexpr int kPairTargetBlocks = 256;
 
 struct PairArgs {
   const float* b;
@@ -292,6 +296,7 @@ struct PairArgs {
   float m1;  // momentum of r_k = p_k + m1 (p_k - p_{k-1})
   int first;
   float m2;  // momentum of r_{k+1} = p_{k+1} + m2 (p_{k+1} - p_k)
+  int zlen;  // planes per chunk
 };
 
 // z = max(b - lambda L(r), 0) on one row of plane zp (the expression order of primal_plane).
@@ -350,6 +355,13 @@ __device__ __forceinline__ void lds_put(float* buf, int row, int nx, int xb, con
         *reinterpret_cast<float4*>(buf + row * nx + xb + i) = float4{v[i], v[i + 1], v[i + 2], v[i + 3]};
       }
     }
+  } else if constexpr (VEC && VX % 2 == 0) {
+    if (xb < nx) {
+#pragma unroll
+      for (int i = 0; i < VX; i += 2) {
+        *reinterpret_cast<float2*>(buf + row * nx + xb + i) = float2{v[i], v[i + 1]};
+      }
+    }
   } else {
 #pragma unroll
     for (int i = 0; i < VX; ++i) {
@@ -369,6 +381,18 @@ __device__ __forceinline__ void lds_get(const float* buf, int row, int nx, int x
         v[i + 1] = q.y;
         v[i + 2] = q.z;
         v[i + 3] = q.w;
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < VX; ++i) v[i] = 0.0f;
+    }
+  } else if constexpr (VEC && VX % 2 == 0) {
+    if (xb < nx) {
+#pragma unroll
+      for (int i = 0; i < VX; i += 2) {
+        const float2 q = *reinterpret_cast<const float2*>(buf + row * nx + xb + i);
+        v[i] = q.x;
+        v[i + 1] = q.y;
       }
     } else {
 #pragma unroll
@@ -400,8 +424,8 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
   const int lane = static_cast<int>(threadIdx.x) & 63;
   const int tile = xcd_tile();
   const int y0 = (tile / chunks) * T;
-  const int z0 = (tile % chunks) * kPairChunk;
-  const int z1 = min(z0 + kPairChunk, a.nz);
+  const int z0 = (tile % chunks) * a.zlen;
+  const int z1 = min(z0 + a.zlen, a.nz);
   const int y = y0 - TOP + w;
   const int xb = lane * VX;
   const int wu = w > 0 ? w - 1 : 0;       // the first and last waves read their own row: their
@@ -713,17 +737,26 @@ void fgp_pair(const float* b, const float* p_k, const float* p_km1, float* p_k1,
               int nz, int ny, int nx, float lambda, float step, float m1, bool first, float m2,
               hipStream_t s) {
   require(fgp_pair_supported(nx), "fgp_pair: rows up to 256 wide");
-  PairArgs a{b, p_k, p_km1, p_k1, p_k2, static_cast<long>(nz) * ny * nx, nz, ny, nx,
-             lambda, step, m1, first ? 1 : 0, m2};
   const int nw = pair_waves(nx);
   const int rows = nw - 4;
-  const int chunks = (nz + kPairChunk - 1) / kPairChunk;
-  const dim3 grid(static_cast<unsigned>(((ny + rows - 1) / rows) * chunks));
+  const int tiles_y = (ny + rows - 1) / rows;
+  int zlen = kPairChunk;
+  while (zlen > kPairChunkMin && tiles_y * ((nz + zlen - 1) / zlen) < kPairTargetBlocks) zlen /= 2;
+  PairArgs a{b, p_k, p_km1, p_k1, p_k2, static_cast<long>(nz) * ny * nx, nz, ny, nx,
+             lambda, step, m1, first ? 1 : 0, m2, zlen};
+  const int chunks = (nz + zlen - 1) / zlen;
+  const dim3 grid(static_cast<unsigned>(tiles_y * chunks));
   const dim3 block(static_cast<unsigned>(nw * 64));
   const std::size_t lds = 4 * static_cast<std::size_t>(nw) * nx * sizeof(float);
   switch (lanes_x(nx)) {
     case 1: hipLaunchKernelGGL((k_fgp_pair<1, 16, false>), grid, block, lds, s, a, chunks); break;
-    case 2: hipLaunchKernelGGL((k_fgp_pair<2, 16, false>), grid, block, lds, s, a, chunks); break;
+    case 2:
+      if (rows_vec<2>(nx)) {
+        hipLaunchKernelGGL((k_fgp_pair<2, 16, true>), grid, block, lds, s, a, chunks);
+      } else {
+        hipLaunchKernelGGL((k_fgp_pair<2, 16, false>), grid, block, lds, s, a, chunks);
+      }
+      break;
     default:
       if (rows_vec<4>(nx)) {
         hipLaunchKernelGGL((k_fgp_pair<4, 12, true>), grid, block, lds, s, a, chunks);
