@@ -71,7 +71,7 @@ __global__ void __launch_bounds__(kThreads) k_fft_lines(LinesArgs a) {
   float2* b0 = smem + n;
   float2* b1 = b0 + a.lines_per_block * n;
   for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = a.tw[i];
-  const long l0 = static_cast<long>(blockIdx.x) * a.lines_per_block;
+  const long l0 = xcd_block() * a.lines_per_block;
   const int nl = static_cast<int>(min(static_cast<long>(a.lines_per_block), a.g.count - l0));
   for (int i = threadIdx.x; i < nl; i += blockDim.x) base[i] = line_base(l0 + i, a.g);
   __syncthreads();
@@ -137,7 +137,7 @@ __global__ void __launch_bounds__(kThreads) k_rfft_rows(RowsArgs a) {
     tw[i] = w;
     if ((i & 1) == 0) twm[i >> 1] = w;
   }
-  const long r0 = static_cast<long>(blockIdx.x) * a.rows_per_block;
+  const long r0 = xcd_block() * a.rows_per_block;
   const int nl = static_cast<int>(min(static_cast<long>(a.rows_per_block), a.rows - r0));
   for (int e = threadIdx.x; e < nl * m; e += blockDim.x) {
     const int line = e / m;
@@ -192,7 +192,7 @@ __global__ void __launch_bounds__(kThreads) k_irfft_rows(RowsArgs a) {
     tw[i] = w;
     if ((i & 1) == 0) twm[i >> 1] = w;
   }
-  const long r0 = static_cast<long>(blockIdx.x) * a.rows_per_block;
+  const long r0 = xcd_block() * a.rows_per_block;
   const int nl = static_cast<int>(min(static_cast<long>(a.rows_per_block), a.rows - r0));
   for (int e = threadIdx.x; e < nl * (m + 1); e += blockDim.x) {
     const int line = e / (m + 1);
@@ -251,7 +251,7 @@ __global__ void __launch_bounds__(kThreads) k_z_roundtrip_restore(ZArgs a) {
   float2* b0 = smem + n;
   float2* b1 = b0 + a.lines_per_block * n;
   for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = a.tw[i];
-  const long l0 = static_cast<long>(blockIdx.x) * a.lines_per_block;
+  const long l0 = xcd_block() * a.lines_per_block;
   const int nl = static_cast<int>(min(static_cast<long>(a.lines_per_block), a.lines - l0));
   const int total = nl * n;
   for (int e = threadIdx.x; e < total; e += blockDim.x) {
@@ -309,7 +309,7 @@ __global__ void __launch_bounds__((Shape4<N1, N2, LINE_FAST>::BLOCK)) k_fft4s_li
   for (int i = tid; i < S::N; i += S::BLOCK) tw[i] = a.tw[i];
   const int line = LINE_FAST ? tid % S::LINES : tid / S::T;
   const int j = LINE_FAST ? tid / S::LINES : tid % S::T;
-  const long l = static_cast<long>(blockIdx.x) * S::LINES + line;
+  const long l = xcd_block() * S::LINES + line;
   const bool valid = l < a.g.count;
   const long base = valid ? line_base(l, a.g) : 0;
   float2 v[N2];
@@ -344,7 +344,7 @@ __global__ void __launch_bounds__((Shape4<N1, N2, true>::BLOCK)) k_fft4s_zrt(ZAr
   for (int i = tid; i < S::N; i += S::BLOCK) tw[i] = a.tw[i];
   const int line = tid % S::LINES;
   const int j = tid / S::LINES;
-  const long l = static_cast<long>(blockIdx.x) * S::LINES + line;
+  const long l = xcd_block() * S::LINES + line;
   const bool valid = l < a.lines;
   float2 v[N2];
   float2 w[N1];
@@ -385,7 +385,7 @@ __global__ void __launch_bounds__((Shape4<N1, N2>::BLOCK)) k_rfft4s_rows(RowsArg
   for (int i = tid; i <= M; i += S::BLOCK) twn[i] = a.tw[i];
   const int line = tid / S::T;
   const int j = tid % S::T;
-  const long row = static_cast<long>(blockIdx.x) * S::LINES + line;
+  const long row = xcd_block() * S::LINES + line;
   const bool valid = row < a.rows;
   float2 v[N2];
   float2 w[N1];
@@ -439,7 +439,7 @@ __global__ void __launch_bounds__((Shape4<N1, N2>::BLOCK)) k_irfft4s_rows(RowsAr
   for (int i = tid; i <= M; i += S::BLOCK) twn[i] = a.tw[i];
   const int line = tid / S::T;
   const int j = tid % S::T;
-  const long row = static_cast<long>(blockIdx.x) * S::LINES + line;
+  const long row = xcd_block() * S::LINES + line;
   const bool valid = row < a.rows;
   float2* xrow = t + line * (M + 1);
   for (int k = j; k <= M; k += S::T) xrow[k] = valid ? a.c[row * a.c_pitch + k] : float2{0.0f, 0.0f};

@@ -14,6 +14,18 @@ namespace qh {
 
 inline constexpr int kMaxFactors = 16;
 
+// Block index with consecutive values on one XCD. Workgroups are dealt to the 8 XCDs in turn,
+// so block b runs on XCD b mod 8; renumbering gives each XCD a contiguous range of blocks, and
+// neighbouring blocks that share 128-byte lines (line batches whose rows are not line aligned,
+// byte arrays read 16 bytes per row) share them through one L2 instead of fetching them once
+// per XCD. The last nb mod 8 blocks keep their numbers.
+__device__ __forceinline__ long xcd_block() {
+  const long nb = gridDim.x;
+  const long b = blockIdx.x;
+  const long full = nb - nb % 8;
+  return b < full ? (b % 8) * (full / 8) + b / 8 : b;
+}
+
 struct FftFactors {
   int n = 0;
   int count = 0;

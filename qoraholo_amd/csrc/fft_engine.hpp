@@ -81,7 +81,7 @@ struct RegLines {
     for (int i = tid; i < N; i += BLOCK) tw[i] = g.tw[i];
     line = LINE_FAST ? tid % LINES : tid / T;
     j = LINE_FAST ? tid / LINES : tid % T;
-    l = static_cast<long>(blockIdx.x) * LINES + line;
+    l = xcd_block() * LINES + line;
     valid = l < g.lines;
   }
 
@@ -174,7 +174,7 @@ struct LdsLines {
     b1 = b0 + g.lines_per_block * n;
     cur = b0;
     for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = g.tw[i];
-    l0 = static_cast<long>(blockIdx.x) * g.lines_per_block;
+    l0 = xcd_block() * g.lines_per_block;
     const long left = g.lines - l0;
     nl = static_cast<int>(left < g.lines_per_block ? left : g.lines_per_block);
     ls = CONTIG ? n : 1;
