@@ -422,9 +422,12 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
   bd.nx = a.nx;
   const int w = static_cast<int>(threadIdx.x) >> 6;
   const int lane = static_cast<int>(threadIdx.x) & 63;
+  // Tiles of one z chunk are consecutive, so the y neighbours that share halo rows run on the
+  // same XCD.
   const int tile = xcd_tile();
-  const int y0 = (tile / chunks) * T;
-  const int z0 = (tile % chunks) * a.zlen;
+  const int tiles_y = static_cast<int>(gridDim.x) / chunks;
+  const int y0 = (tile % tiles_y) * T;
+  const int z0 = (tile / tiles_y) * a.zlen;
   const int z1 = min(z0 + a.zlen, a.nz);
   const int y = y0 - TOP + w;
   const int xb = lane * VX;
