@@ -51,28 +51,43 @@ def fgp_traffic(kernel: str):
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(w):
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(w, runs=3):
     """Times the oracle (the C++/OpenMP restatement of the reference's CPU path) on the same
-    inputs for one whole reconstruction, and returns its volume for the parity check."""
+    inputs for whole reconstructions (median of `runs`), and returns its volume for the parity
+    check."""
     import oracle
 
     g = w.geometry
     geom = (g.wavelength, g.n_medium, g.na, g.pixel)
-    cy, cx = int(round(w.center[0])), int(round(w.center[1]))
-    t0 = time.perf_counter()
-    ri, _ = oracle.pipeline(w.holograms, w.backgrounds, cy, cx, w.radius, N, w.illuminations, geom,
-                            iterations=ITERATIONS, solver="tv", tv_weight=TV_WEIGHT,
-                            tv_inner=INNER)
-    secs = time.perf_counter() - t0
-    log(f"cpu baseline: one reconstruction in {secs:.2f} s")
+    cy, cx = (int(np.sign(c) * np.floor(abs(c) + 0.5)) for c in w.center)  # lround
+    times = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        ri, _ = oracle.pipeline(w.holograms, w.backgrounds, cy, cx, w.radius, N, w.illuminations,
+                                geom, iterations=ITERATIONS, solver="tv", tv_weight=TV_WEIGHT,
+                                tv_inner=INNER)
+        times.append(time.perf_counter() - t0)
+    secs = float(np.median(times))
+    log(f"cpu baseline: reconstructions in {', '.join(f'{t:.2f}' for t in times)} s")
     cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
     return {
         "value": round(1.0 / secs, 4),
         "unit": "recon/s",
         "cores": cores,
         "kind": "port",
-        "sample": (f"one full reconstruction of the same workload by the oracle (C++/OpenMP "
-                   f"restatement of the reference CPU path, own FFT), {secs:.2f} s"),
+        "sample": (f"full reconstructions of the same workload by the oracle (C++/OpenMP "
+                   f"restatement of the reference CPU path, own FFT), median of {runs}: "
+                   f"{secs:.2f} s on {cores} threads of {cpu_model()}"),
     }, ri
 
 
@@ -535,6 +550,10 @@ def main() -> int:
         "min_bytes": min_bytes,
         "min_bytes_gbs": round(min_bytes / (fgp_avg_ms * 1e-3) / 1e9, 1),
     }
+    if roofline["traffic"]:
+        # The physical rate: measured HBM bytes of one launch over its duration.
+        roofline["traffic_gbs"] = round(roofline["traffic"] / (fgp_avg_ms * 1e-3) / 1e9, 1)
+        roofline["traffic_frac"] = round(roofline["traffic_gbs"] / HBM_PEAK_GBS, 4)
     stages = {k: round(v, 3) for k, v in t.items() if k not in ("fgp_steps",)}
 
     result = {

@@ -389,22 +389,33 @@ __global__ void __launch_bounds__((Shape4<N1, N2>::BLOCK)) k_rfft4s_rows(RowsArg
   const bool valid = row < a.rows;
   float2 v[N2];
   float2 w[N1];
+  // All loads of a thread are issued before the first use (the input operation is uniform over
+  // the grid), so their latencies overlap.
+  const bool live = valid && j < N1;
+  const bool momentum = a.op.in == RealIn::momentum;
+  const long at0 = row * a.x_pitch + 2 * j;
+  float2 q[N2];
 #pragma unroll
   for (int n2 = 0; n2 < N2; ++n2) {
-    float2 z{0.0f, 0.0f};
-    if (valid && j < N1) {
-      const long at = row * a.x_pitch + 2 * (j + N1 * n2);
-      z = *reinterpret_cast<const float2*>(a.x + at);
-      if (a.op.in == RealIn::nonneg) {
-        z.x = z.x < 0.0f ? 0.0f : z.x;
-        z.y = z.y < 0.0f ? 0.0f : z.y;
-      } else if (a.op.in == RealIn::momentum) {
-        const float2 q = *reinterpret_cast<const float2*>(a.x2 + at);
-        z.x = z.x + a.op.a * (z.x - q.x);
-        z.y = z.y + a.op.a * (z.y - q.y);
-      }
+    v[n2] = live ? *reinterpret_cast<const float2*>(a.x + at0 + 2 * N1 * n2) : float2{0.0f, 0.0f};
+  }
+#pragma unroll
+  for (int n2 = 0; n2 < N2; ++n2) {
+    q[n2] = (live && momentum) ? *reinterpret_cast<const float2*>(a.x2 + at0 + 2 * N1 * n2)
+                               : float2{0.0f, 0.0f};
+  }
+  if (a.op.in == RealIn::nonneg) {
+#pragma unroll
+    for (int n2 = 0; n2 < N2; ++n2) {
+      v[n2].x = v[n2].x < 0.0f ? 0.0f : v[n2].x;
+      v[n2].y = v[n2].y < 0.0f ? 0.0f : v[n2].y;
     }
-    v[n2] = z;
+  } else if (momentum) {
+#pragma unroll
+    for (int n2 = 0; n2 < N2; ++n2) {
+      v[n2].x = v[n2].x + a.op.a * (v[n2].x - q[n2].x);
+      v[n2].y = v[n2].y + a.op.a * (v[n2].y - q[n2].y);
+    }
   }
   __syncthreads();
   fft_4step<N1, N2, S::LINES, false>(v, w, j, line, t, twm, false);
@@ -442,7 +453,21 @@ __global__ void __launch_bounds__((Shape4<N1, N2>::BLOCK)) k_irfft4s_rows(RowsAr
   const long row = xcd_block() * S::LINES + line;
   const bool valid = row < a.rows;
   float2* xrow = t + line * (M + 1);
-  for (int k = j; k <= M; k += S::T) xrow[k] = valid ? a.c[row * a.c_pitch + k] : float2{0.0f, 0.0f};
+  {
+    // Bins 0 .. M of the row, loaded together before they are staged in LDS.
+    constexpr int kPer = (M + S::T) / S::T;
+    float2 c[kPer];
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+      const int k = j + i * S::T;
+      c[i] = (valid && k <= M) ? a.c[row * a.c_pitch + k] : float2{0.0f, 0.0f};
+    }
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+      const int k = j + i * S::T;
+      if (k <= M) xrow[k] = c[i];
+    }
+  }
   __syncthreads();
   float2 v[N2];
   float2 w[N1];
