@@ -514,6 +514,15 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
   // The loads of the next plane are issued before the current plane's last dual step, so that
   // their latency overlaps that work instead of stalling the block at the next barrier; r_k is
   // formed from them at the top of the next iteration.
+  // What each wave's row feeds: the output rows (waves 2 .. NW-3) need p_{k+2}, which needs
+  // z_{k+1} one row further down (waves 2 .. NW-2), which needs r_{k+1} one row up (waves
+  // 1 .. NW-2), which needs z_k one row down (waves 1 .. NW-1); wave 0 only supplies r_k. A
+  // wave skips the quantities no output depends on (wave-uniform branches; the variant with
+  // element-wise row access keeps computing them, which holds it below the register limit).
+  const bool need_zk = !VEC || w >= 1;
+  const bool need_pk1 = !VEC || (w >= 1 && w <= NW - 2);
+  const bool need_zq = !VEC || (w >= 2 && w <= NW - 2);
+  const bool need_pk2 = !VEC || (w >= TOP && w < TOP + T);
   float pk2[3][VX], pm2[3][VX], b2[VX];
   load_raw(z0 - pre + 2, pk2, pm2, b2);
   for (int t = z0 - pre; t < z1; ++t) {
@@ -522,30 +531,33 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
     extrapolate_k(pk2, pm2, rk2);
     lds_put<VX, VEC>(ry, w, nx, xb, rk2[1]);
     __syncthreads();  // (1) r_k of plane f visible
-    float zk2[VX];
-    {
+    float zk2[VX] = {};
+    if (need_zk) {
       float up[VX];
       lds_get<VX, VEC>(ry, wu, nx, xb, up);
       pair_primal<VX>(a, f, y, xb, rk2, rk1[0], up, b2, zk2);
     }
     // p_{k+1} and r_{k+1} on plane t + 1.
-    float pq1[3][VX], rq1[3][VX];
-    pair_dual<VX>(a, t + 1, y, xb, zk1, zk2, zdn_k, rk1, pq1);
-    if (a.p_k1 && out_row && t + 1 >= z0 && t + 1 < z1) {
+    float rq1[3][VX] = {};
+    if (need_pk1) {
+      float pq1[3][VX];
+      pair_dual<VX>(a, t + 1, y, xb, zk1, zk2, zdn_k, rk1, pq1);
+      if (a.p_k1 && out_row && t + 1 >= z0 && t + 1 < z1) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) store_row<VX, VEC>(a.p_k1 + c * a.vol, bd, t + 1, y, xb, pq1[c]);
-    }
+        for (int c = 0; c < 3; ++c) store_row<VX, VEC>(a.p_k1 + c * a.vol, bd, t + 1, y, xb, pq1[c]);
+      }
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
+      for (int c = 0; c < 3; ++c) {
 #pragma unroll
-      for (int v = 0; v < VX; ++v) rq1[c][v] = pq1[c][v] + a.m2 * (pq1[c][v] - pk1[c][v]);
+        for (int v = 0; v < VX; ++v) rq1[c][v] = pq1[c][v] + a.m2 * (pq1[c][v] - pk1[c][v]);
+      }
     }
     lds_put<VX, VEC>(zk, w, nx, xb, zk2);
     lds_put<VX, VEC>(rqy, w, nx, xb, rq1[1]);
     __syncthreads();  // (2) z_k of plane f and r_{k+1} of plane t + 1 visible
     lds_get<VX, VEC>(zk, wd, nx, xb, zdn_k);  // for the next plane's dual step
-    float zq1[VX];
-    {
+    float zq1[VX] = {};
+    if (need_zq) {
       float up[VX];
       lds_get<VX, VEC>(rqy, wu, nx, xb, up);
       pair_primal<VX>(a, t + 1, y, xb, rq1, rq0[0], up, b1, zq1);
@@ -555,7 +567,7 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
     float pk3[3][VX], pm3[3][VX], b3[VX];
     load_raw(f + 1, pk3, pm3, b3);
     // p_{k+2} on plane t.
-    if (t >= z0) {
+    if (t >= z0 && need_pk2) {
       float p2[3][VX];
       pair_dual<VX>(a, t, y, xb, zq0, zq1, zdn_q, rq0, p2);
       if (out_row) {
