@@ -740,18 +740,19 @@ void tv_primal(const float* b, const float* p, int nz, int ny, int nx, float lam
 
 namespace {
 
-// Rows per block of the two-step kernel: NW waves, NW - 4 output rows. Four row buffers of
-// NW x nx floats stay within 64 KiB of LDS.
-int pair_waves(int nx) { return nx <= 128 ? 16 : 12; }
+// Rows per block of the two-step kernel: NW waves, NW - 4 output rows, as many waves as the
+// registers of VX floats per lane leave room for in one block per CU (rows of 384 take 6 floats
+// per lane and 8 waves). Four row buffers of NW x nx floats stay within 64 KiB of LDS.
+int pair_waves(int nx) { return nx <= 128 ? 16 : (nx <= 256 ? 12 : 8); }
 
 }  // namespace
 
-bool fgp_pair_supported(int nx) { return nx >= 1 && nx <= 256; }
+bool fgp_pair_supported(int nx) { return nx >= 1 && nx <= 384; }
 
 void fgp_pair(const float* b, const float* p_k, const float* p_km1, float* p_k1, float* p_k2,
               int nz, int ny, int nx, float lambda, float step, float m1, bool first, float m2,
               hipStream_t s) {
-  require(fgp_pair_supported(nx), "fgp_pair: rows up to 256 wide");
+  require(fgp_pair_supported(nx), "fgp_pair: rows up to 384 wide");
   const int nw = pair_waves(nx);
   const int rows = nw - 4;
   const int tiles_y = (ny + rows - 1) / rows;
@@ -772,11 +773,18 @@ void fgp_pair(const float* b, const float* p_k, const float* p_km1, float* p_k1,
         hipLaunchKernelGGL((k_fgp_pair<2, 16, false>), grid, block, lds, s, a, chunks);
       }
       break;
-    default:
+    case 4:
       if (rows_vec<4>(nx)) {
         hipLaunchKernelGGL((k_fgp_pair<4, 12, true>), grid, block, lds, s, a, chunks);
       } else {
         hipLaunchKernelGGL((k_fgp_pair<4, 12, false>), grid, block, lds, s, a, chunks);
+      }
+      break;
+    default:
+      if (rows_vec<6>(nx)) {
+        hipLaunchKernelGGL((k_fgp_pair<6, 8, true>), grid, block, lds, s, a, chunks);
+      } else {
+        hipLaunchKernelGGL((k_fgp_pair<6, 8, false>), grid, block, lds, s, a, chunks);
       }
       break;
   }

@@ -187,12 +187,14 @@ def test_tv_prox_partial_tiles(device, oracle):
 
 @pytest.mark.parametrize("shape,inner", [((62, 60, 60), 10), ((40, 33, 130), 7), ((70, 128, 100), 4),
                                          ((33, 77, 256), 5), ((256, 256, 256), 10),
-                                         ((20, 30, 300), 3), ((44, 50, 128), 6)])
+                                         ((20, 30, 300), 3), ((44, 50, 128), 6),
+                                         ((96, 100, 384), 4), ((12, 18, 302), 5),
+                                         ((10, 12, 400), 3)])
 def test_tv_two_step_passes_equal_single_steps(device, shape, inner):
     """Two FGP steps per pass give the bits of one step per pass: partial row tiles and z
-    chunks, odd inner counts (a single last step), row widths for 1, 2 and 4 floats per lane
-    with and without whole-vector access, the full 256^3 volume, a small volume (short z
-    chunks), and a width the paired kernel does not take (single steps)."""
+    chunks, odd inner counts (a single last step), row widths for 1, 2, 4 and 6 floats per
+    lane with and without whole-vector access, the full 256^3 volume, 384-wide rows, a small
+    volume (short z chunks), and a width the paired kernel does not take (single steps)."""
     from qoraholo_amd import _native
 
     rng = np.random.default_rng(shape[0])
