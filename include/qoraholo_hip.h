@@ -138,6 +138,18 @@ int qh_resolve_crop(float center_y, float center_x, float radius, int output_siz
 int qh_circular_scan(int count, float na_illumination, const qh_geometry* geometry,
                      qh_illumination* out);
 
+/* src/cpu/field.cpp:62-74 (field_ratio, host-only in the reference): out = sample / background
+ * with std::complex<float> division, per pixel of `count` planes of `plane` values; bcount is 1
+ * (one background for every plane) or count. Runs on the library's copy threads. */
+int qh_field_ratio(const qh_c64* sample, int count, const qh_c64* background, int bcount,
+                   int plane, qh_c64* out);
+
+/* No counterpart in the reference (its bindings copy arrays with one thread,
+ * python/bindings.cpp:27-32): dst = src for `bytes` bytes of host memory, spread over the
+ * library's copy threads (half the CPUs the process may use, at most 8). The copies into and
+ * out of pinned staging buffers of every host-array entry point use the same threads. */
+int qh_host_copy(void* dst, const void* src, size_t bytes);
+
 /* The noise stream of src/phantom.cpp:200-201: std::mt19937_64(seed) drawn through
  * std::normal_distribution<float>(0, sigma), `count` values in order. */
 int qh_gaussian_noise(uint64_t seed, float sigma, size_t count, float* out);
@@ -205,6 +217,11 @@ float* qh_hip_pipeline_input(qh_pipeline* p);
 const float* qh_hip_pipeline_output(const qh_pipeline* p);
 /* Reconstructs from input() into output(); QH_ERR_LOGIC without a background. */
 int qh_hip_pipeline_process(qh_pipeline* p);
+/* Pipeline::run (include/qoraholo/pipeline.hpp:72): the same from a caller's pageable
+ * buffer of holograms (count, N, N) into a caller's volume (nz, n, n). The holograms pass
+ * through input() in pieces whose host copies overlap their uploads, and the volume leaves in
+ * parts whose host copies overlap the DMA of the next part; output() holds the volume too. */
+int qh_hip_pipeline_run(qh_pipeline* p, const float* holograms, float* ri);
 /* The same from holograms already in device memory into a device volume, on `stream`
  * (a hipStream_t, used as given: NULL is the null stream, which is also torch's default
  * stream). Returns once the work is queued; the caller synchronises the stream before reading

@@ -239,12 +239,17 @@ def extract_fields(holograms, center=None, radius: float = 0.0, dc_exclusion: fl
 
 
 def field_ratio(sample, background):
-    """sample / background; background is one field or one per sample field (host)."""
+    """sample / background; background is one field or one per sample field (host, on the
+    library's copy threads, with the reference's std::complex<float> division)."""
     s, single = _stack(sample, np.complex64)
     b, _ = _stack(background, np.complex64)
     if s.shape[1:] != b.shape[1:] or (b.shape[0] != 1 and b.shape[0] != s.shape[0]):
         raise ValueError("field_ratio: background shape does not match")
-    return _unstack((s / b).astype(np.complex64), single)
+    out = np.empty_like(s)
+    _native.check(_native.lib().qh_field_ratio(_native.ptr(s), s.shape[0], _native.ptr(b),
+                                               b.shape[0], s.shape[1] * s.shape[2],
+                                               _native.ptr(out)))
+    return _unstack(out, single)
 
 
 def unwrap(phase, backend: str = "hip"):
@@ -424,13 +429,15 @@ class Pipeline:
         _native.check(_native.lib().qh_hip_pipeline_process(self._handle))
 
     def run(self, holograms) -> np.ndarray:
-        """Sample holograms (count, N, N) to the index volume (nz, n, n)."""
+        """Sample holograms (count, N, N) to the index volume (nz, n, n). The copies through the
+        page-locked buffers run on the library's copy threads and overlap the transfers."""
         h = np.ascontiguousarray(holograms, dtype=np.float32)
         if h.shape != (self.count, self.hologram_size, self.hologram_size):
             raise ValueError("Pipeline::run: shape must be (count, N, N)")
-        self.input()[...] = h
-        self.process()
-        return self.output().copy()
+        out = np.empty(self.output_shape, np.float32)
+        _native.check(_native.lib().qh_hip_pipeline_run(self._handle, _native.ptr(h),
+                                                         _native.ptr(out)))
+        return out
 
     def run_device(self, holograms, out, stream=None) -> None:
         """Holograms in device memory -> index volume in device memory, queued on `stream`

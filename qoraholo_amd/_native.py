@@ -87,6 +87,8 @@ _SIGNATURES = {
     "qh_resolve_crop": (c_int, [c_float, c_float, c_float, c_int, c_int, POINTER(CropPlan_t)]),
     "qh_circular_scan": (c_int, [c_int, c_float, POINTER(Geometry_t), c_void_p]),
     "qh_gaussian_noise": (c_int, [c_uint64, c_float, c_size_t, c_void_p]),
+    "qh_host_copy": (c_int, [c_void_p, c_void_p, c_size_t]),
+    "qh_field_ratio": (c_int, [c_void_p, c_int, c_void_p, c_int, c_int, c_void_p]),
     "qh_hip_extract_fields": (c_int, [c_void_p, c_int, c_int, POINTER(CropPlan_t), c_void_p]),
     "qh_hip_unwrap": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p]),
     "qh_hip_rytov": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p]),
@@ -104,6 +106,7 @@ _SIGNATURES = {
     "qh_hip_pipeline_input": (c_void_p, [c_void_p]),
     "qh_hip_pipeline_output": (c_void_p, [c_void_p]),
     "qh_hip_pipeline_process": (c_int, [c_void_p]),
+    "qh_hip_pipeline_run": (c_int, [c_void_p, c_void_p, c_void_p]),
     "qh_hip_pipeline_process_device": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "qh_hip_pipeline_filled": (c_double, [c_void_p]),
     "qh_hip_pipeline_timings": (c_int, [c_void_p, POINTER(PipelineTimings_t)]),

@@ -7,6 +7,7 @@
 #include <complex>
 #include <numbers>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <new>
 #include <random>
@@ -17,6 +18,7 @@
 #include "bpm.hpp"
 #include "fft.hpp"
 #include "pipeline.hpp"
+#include "hostio.hpp"
 #include "runtime.hpp"
 #include "server.hpp"
 #include "stages.hpp"
@@ -147,10 +149,10 @@ std::vector<std::array<float, 5>> device_sidebands(const float* holo, int count,
   DeviceBuffer<float> dh(plane * count);
   DeviceBuffer<float2> half(static_cast<std::size_t>(count) * n * (n / 2 + 1));
   DeviceBuffer<SidebandPeak> peaks(count);
-  check(hipMemcpyAsync(dh.get(), holo, dh.bytes(), hipMemcpyHostToDevice, s), "upload");
+  upload(dh.get(), holo, dh.bytes(), s);
   sideband_peaks(dh.get(), count, n, dc_exclusion, half.get(), peaks.get(), s);
   std::vector<SidebandPeak> h(count);
-  check(hipMemcpyAsync(h.data(), peaks.get(), peaks.bytes(), hipMemcpyDeviceToHost, s), "download");
+  download(h.data(), peaks.get(), peaks.bytes(), s);
   check(hipStreamSynchronize(s), "find_sideband");
   std::vector<std::array<float, 5>> out(count);
   for (int a = 0; a < count; ++a) {
@@ -182,6 +184,36 @@ BpmTables checked_bpm_tables(int nz, int n, int count, const qh_illumination* il
   require(o->chunk >= 0 && o->checkpoint >= 0, w + ": chunk and checkpoint must not be negative");
   require_fft(n, who);
   return bpm_tables(n, nz, illum, count, *g, o->obliquity != 0);
+}
+
+// Complex float division as the reference's host build evaluates std::complex<float> / (GCC's
+// runtime on x86-64, which the reference's CPU loop calls): numerator and denominator in double,
+// the simple formula (double carries the range and precision Smith's scaling would otherwise
+// buy), each part rounded once to float; NaN results recovered as in C99 Annex G.3.
+std::complex<float> cdiv(std::complex<float> num, std::complex<float> den) {
+  float a = num.real(), b = num.imag(), c = den.real(), d = den.imag();
+  const double aa = a, bb = b, cc = c, dd = d;
+  const double denom = cc * cc + dd * dd;
+  float x = static_cast<float>((aa * cc + bb * dd) / denom);
+  float y = static_cast<float>((bb * cc - aa * dd) / denom);
+  if (std::isnan(x) && std::isnan(y)) {
+    const float inf = std::numeric_limits<float>::infinity();
+    if (c == 0.0f && d == 0.0f && (!std::isnan(a) || !std::isnan(b))) {
+      x = std::copysign(inf, c) * a;
+      y = std::copysign(inf, c) * b;
+    } else if ((std::isinf(a) || std::isinf(b)) && std::isfinite(c) && std::isfinite(d)) {
+      a = std::copysign(std::isinf(a) ? 1.0f : 0.0f, a);
+      b = std::copysign(std::isinf(b) ? 1.0f : 0.0f, b);
+      x = inf * (a * c + b * d);
+      y = inf * (b * c - a * d);
+    } else if ((std::isinf(c) || std::isinf(d)) && std::isfinite(a) && std::isfinite(b)) {
+      c = std::copysign(std::isinf(c) ? 1.0f : 0.0f, c);
+      d = std::copysign(std::isinf(d) ? 1.0f : 0.0f, d);
+      x = 0.0f * (a * c + b * d);
+      y = 0.0f * (b * c - a * d);
+    }
+  }
+  return {x, y};
 }
 
 }  // namespace
@@ -264,6 +296,40 @@ int qh_circular_scan(int count, float na_illumination, const qh_geometry* geomet
   });
 }
 
+int qh_field_ratio(const qh_c64* sample, int count, const qh_c64* background, int bcount,
+                   int plane, qh_c64* out) {
+  return guarded([&] {
+    require(sample && background && out, "field_ratio: null argument");
+    require(count >= 1 && plane >= 1 && (bcount == 1 || bcount == count),
+            "field_ratio: background shape does not match");
+    using C = std::complex<float>;
+    const auto* s = reinterpret_cast<const C*>(sample);
+    const auto* b = reinterpret_cast<const C*>(background);
+    auto* o = reinterpret_cast<C*>(out);
+    const std::size_t total = static_cast<std::size_t>(count) * plane;
+    const int parts = static_cast<int>(
+        std::clamp<std::size_t>(total >> 16, 1, static_cast<std::size_t>(host_copy_threads())));
+    const std::size_t step = (total + parts - 1) / parts;
+    parallel_for(parts, [&](int p) {
+      const std::size_t end = std::min(total, (p + 1) * step);
+      for (std::size_t i = p * step; i < end;) {
+        const std::size_t z = i / plane;
+        const std::size_t stop = std::min(end, (z + 1) * plane);
+        const C* bz = b + (bcount == 1 ? 0 : z * plane);
+        const std::size_t z0 = z * plane;
+        for (; i < stop; ++i) o[i] = cdiv(s[i], bz[i - z0]);
+      }
+    });
+  });
+}
+
+int qh_host_copy(void* dst, const void* src, size_t bytes) {
+  return guarded([&] {
+    require(bytes == 0 || (dst && src), "host_copy: null buffer");
+    host_copy(dst, src, bytes);
+  });
+}
+
 int qh_gaussian_noise(uint64_t seed, float sigma, size_t count, float* out) {
   return guarded([&] {
     require(out != nullptr || count == 0, "gaussian_noise: null output");
@@ -289,12 +355,12 @@ int qh_hip_extract_fields(const float* holograms, int count, int n, const qh_cro
     DeviceBuffer<float> holo(count * image);
     DeviceBuffer<float2> half(static_cast<std::size_t>(count) * n * (n / 2 + 1));
     DeviceBuffer<float2> out(static_cast<std::size_t>(count) * m * m);
-    check(hipMemcpyAsync(holo.get(), holograms, holo.bytes(), hipMemcpyHostToDevice, s), "upload");
+    upload(holo.get(), holograms, holo.bytes(), s);
     hologram_spectra(holo.get(), count, n, half.get(), s);
     const CropPlan cp{plan->cy, plan->cx, plan->radius, m};
     crop_fields(half.get(), count, n, cp, out.get(), s);
     fields_inverse(out.get(), count, m, 1.0f / static_cast<float>(image), true, s);
-    check(hipMemcpyAsync(fields, out.get(), out.bytes(), hipMemcpyDeviceToHost, s), "download");
+    download(fields, out.get(), out.bytes(), s);
     check(hipStreamSynchronize(s), "extract_fields");
   });
 }
@@ -312,9 +378,9 @@ int qh_hip_unwrap(const float* wrapped, int nz, int ny, int nx, float* out) {
     DeviceBuffer<float> psi(total);
     DeviceBuffer<float> phi(total);
     Unwrapper u(nz, ny, nx);
-    check(hipMemcpyAsync(psi.get(), wrapped, psi.bytes(), hipMemcpyHostToDevice, s), "upload");
+    upload(psi.get(), wrapped, psi.bytes(), s);
     u.unwrap(psi.get(), phi.get(), s);
-    check(hipMemcpyAsync(out, phi.get(), phi.bytes(), hipMemcpyDeviceToHost, s), "download");
+    download(out, phi.get(), phi.bytes(), s);
     check(hipStreamSynchronize(s), "unwrap");
   });
 }
@@ -333,10 +399,10 @@ int qh_hip_rytov(const qh_c64* ratio, int nz, int ny, int nx, qh_c64* out) {
     DeviceBuffer<float> psi(total);
     DeviceBuffer<float> logamp(total);
     Unwrapper u(nz, ny, nx);
-    check(hipMemcpyAsync(r.get(), ratio, r.bytes(), hipMemcpyHostToDevice, s), "upload");
+    upload(r.get(), ratio, r.bytes(), s);
     split_ratio(r.get(), static_cast<long>(total), psi.get(), logamp.get(), s);
     u.rytov(psi.get(), logamp.get(), r.get(), s);
-    check(hipMemcpyAsync(out, r.get(), r.bytes(), hipMemcpyDeviceToHost, s), "download");
+    download(out, r.get(), r.bytes(), s);
     check(hipStreamSynchronize(s), "rytov");
   });
 }
@@ -373,14 +439,13 @@ int qh_hip_reconstruct(const qh_c64* fields, int count, int n, const qh_illumina
     StageClock clock(kOdtMarks + 3);
     const int start = kOdtMarks, uploaded = kOdtMarks + 1, downloaded = kOdtMarks + 2;
     clock.mark(start, s);
-    check(hipMemcpyAsync(f.get(), fields, f.bytes(), hipMemcpyHostToDevice, s), "upload");
+    upload(f.get(), fields, f.bytes(), s);
     clock.mark(uploaded, s);
     solver.run(f.get(), s, &clock, 0);
     const std::size_t vol = plane * g.nz;
     unsigned long long nfilled = 0;
-    check(hipMemcpyAsync(ri, solver.index(), vol * sizeof(float), hipMemcpyDeviceToHost, s), "download");
-    check(hipMemcpyAsync(&nfilled, solver.filled_device(), sizeof nfilled, hipMemcpyDeviceToHost, s),
-          "download");
+    download(ri, solver.index(), vol * sizeof(float), s);
+    download(&nfilled, solver.filled_device(), sizeof nfilled, s);
     clock.mark(downloaded, s);
     check(hipStreamSynchronize(s), "reconstruct");
     if (filled) *filled = static_cast<double>(nfilled) / static_cast<double>(vol);
@@ -423,16 +488,13 @@ int qh_hip_map_kspace(const qh_c64* fields, int count, int n, const qh_illuminat
     solver.set_illuminations(il.data());
     const std::size_t plane = static_cast<std::size_t>(n) * n;
     DeviceBuffer<float2> f(plane * count);
-    check(hipMemcpyAsync(f.get(), fields, f.bytes(), hipMemcpyHostToDevice, s), "upload");
+    upload(f.get(), fields, f.bytes(), s);
     solver.map_only(f.get(), s);
     const std::size_t vol = plane * g.nz;
     unsigned long long nfilled = 0;
-    check(hipMemcpyAsync(potential, solver.potential(), vol * sizeof(float2), hipMemcpyDeviceToHost, s),
-          "download");
-    check(hipMemcpyAsync(counts, solver.counts(), vol * sizeof(uint16_t), hipMemcpyDeviceToHost, s),
-          "download");
-    check(hipMemcpyAsync(&nfilled, solver.filled_device(), sizeof nfilled, hipMemcpyDeviceToHost, s),
-          "download");
+    download(potential, solver.potential(), vol * sizeof(float2), s);
+    download(counts, solver.counts(), vol * sizeof(uint16_t), s);
+    download(&nfilled, solver.filled_device(), sizeof nfilled, s);
     check(hipStreamSynchronize(s), "map_kspace");
     if (filled) *filled = static_cast<double>(nfilled) / static_cast<double>(vol);
   });
@@ -475,6 +537,14 @@ int qh_hip_pipeline_process(qh_pipeline* p) {
   return guarded([&] {
     require(p != nullptr, "Pipeline: null handle");
     p->impl->process();
+  });
+}
+
+int qh_hip_pipeline_run(qh_pipeline* p, const float* holograms, float* ri) {
+  return guarded([&] {
+    require(p != nullptr, "Pipeline: null handle");
+    require(holograms != nullptr && ri != nullptr, "Pipeline: null buffer");
+    p->impl->run(holograms, ri);
   });
 }
 
@@ -587,9 +657,9 @@ int qh_hip_fft_c2c(qh_c64* data, int d0, int d1, int d2, unsigned axes, int forw
     require_device();
     const hipStream_t s = call_stream();
     DeviceBuffer<float2> d(static_cast<std::size_t>(d0) * d1 * d2);
-    check(hipMemcpyAsync(d.get(), data, d.bytes(), hipMemcpyHostToDevice, s), "upload");
+    upload(d.get(), data, d.bytes(), s);
     fft3(d.get(), d0, d1, d2, axes, forward == 0, scale, s);
-    check(hipMemcpyAsync(data, d.get(), d.bytes(), hipMemcpyDeviceToHost, s), "download");
+    download(data, d.get(), d.bytes(), s);
     check(hipStreamSynchronize(s), "fft");
   });
 }
@@ -606,9 +676,9 @@ int qh_hip_rfft3(const float* x, int nz, int ny, int nx, qh_c64* out) {
     const std::size_t half = static_cast<std::size_t>(nz) * ny * (nx / 2 + 1);
     DeviceBuffer<float> dx(vol);
     DeviceBuffer<float2> dh(half);
-    check(hipMemcpyAsync(dx.get(), x, dx.bytes(), hipMemcpyHostToDevice, s), "upload");
+    upload(dx.get(), x, dx.bytes(), s);
     rfft3(dx.get(), dh.get(), nz, ny, nx, s);
-    check(hipMemcpyAsync(out, dh.get(), dh.bytes(), hipMemcpyDeviceToHost, s), "download");
+    download(out, dh.get(), dh.bytes(), s);
     check(hipStreamSynchronize(s), "rfft3");
   });
 }
@@ -625,9 +695,9 @@ int qh_hip_irfft3(const qh_c64* in, int nz, int ny, int nx, float scale, float* 
     const std::size_t half = static_cast<std::size_t>(nz) * ny * (nx / 2 + 1);
     DeviceBuffer<float> dx(vol);
     DeviceBuffer<float2> dh(half);
-    check(hipMemcpyAsync(dh.get(), in, dh.bytes(), hipMemcpyHostToDevice, s), "upload");
+    upload(dh.get(), in, dh.bytes(), s);
     irfft3(dh.get(), dx.get(), nz, ny, nx, scale, s);
-    check(hipMemcpyAsync(x, dx.get(), dx.bytes(), hipMemcpyDeviceToHost, s), "download");
+    download(x, dx.get(), dx.bytes(), s);
     check(hipStreamSynchronize(s), "irfft3");
   });
 }
@@ -684,9 +754,9 @@ int qh_hip_bpm_forward(const float* x, int nz, int n, const qh_illumination* ill
     const std::size_t vol = static_cast<std::size_t>(nz) * n * n;
     DeviceBuffer<float> dx(vol);
     DeviceBuffer<float2> dy(static_cast<std::size_t>(count) * n * n);
-    check(hipMemcpyAsync(dx.get(), x, dx.bytes(), hipMemcpyHostToDevice, s), "upload");
+    upload(dx.get(), x, dx.bytes(), s);
     m.forward(dx.get(), dy.get(), s);
-    check(hipMemcpyAsync(out, dy.get(), dy.bytes(), hipMemcpyDeviceToHost, s), "download");
+    download(out, dy.get(), dy.bytes(), s);
     check(hipStreamSynchronize(s), "bpm_forward");
   });
 }
@@ -703,10 +773,10 @@ int qh_hip_bpm_vjp(const float* x, const qh_c64* r, int nz, int n, const qh_illu
     const std::size_t vol = static_cast<std::size_t>(nz) * n * n;
     DeviceBuffer<float> dx(vol), dg(vol);
     DeviceBuffer<float2> dr(static_cast<std::size_t>(count) * n * n);
-    check(hipMemcpyAsync(dx.get(), x, dx.bytes(), hipMemcpyHostToDevice, s), "upload");
-    check(hipMemcpyAsync(dr.get(), r, dr.bytes(), hipMemcpyHostToDevice, s), "upload");
+    upload(dx.get(), x, dx.bytes(), s);
+    upload(dr.get(), r, dr.bytes(), s);
     m.vjp(dx.get(), dr.get(), dg.get(), s);
-    check(hipMemcpyAsync(grad, dg.get(), dg.bytes(), hipMemcpyDeviceToHost, s), "download");
+    download(grad, dg.get(), dg.bytes(), s);
     check(hipStreamSynchronize(s), "bpm_vjp");
   });
 }
@@ -728,8 +798,8 @@ int qh_hip_bpm_reconstruct(const qh_c64* fields, int count, int n, const qh_illu
     const std::size_t vol = static_cast<std::size_t>(nz) * n * n;
     DeviceBuffer<float> dx(vol);
     DeviceBuffer<float2> df(static_cast<std::size_t>(count) * n * n);
-    check(hipMemcpyAsync(dx.get(), x, dx.bytes(), hipMemcpyHostToDevice, s), "upload");
-    check(hipMemcpyAsync(df.get(), fields, df.bytes(), hipMemcpyHostToDevice, s), "upload");
+    upload(dx.get(), x, dx.bytes(), s);
+    upload(df.get(), fields, df.bytes(), s);
     BpmSolveConfig c;
     c.iterations = solve->iterations;
     c.batch = solve->batch;
@@ -738,7 +808,7 @@ int qh_hip_bpm_reconstruct(const qh_c64* fields, int count, int n, const qh_illu
     c.tv_inner = solve->tv_inner;
     std::vector<double> l(std::max(c.iterations, 1));
     bpm_solve(m, df.get(), dx.get(), c, l.data(), s);
-    check(hipMemcpyAsync(x, dx.get(), dx.bytes(), hipMemcpyDeviceToHost, s), "download");
+    download(x, dx.get(), dx.bytes(), s);
     check(hipStreamSynchronize(s), "bpm_reconstruct");
     if (c.iterations > 0) std::memcpy(loss, l.data(), c.iterations * sizeof(double));
   });
@@ -803,7 +873,7 @@ int qh_hip_bpm_gradient_dev(qh_bpm_plan* plan, const float* x, const qh_c64* fie
     bpm_scale(grad, static_cast<long>(m.nz()) * m.n() * m.n(),
               1.0f / static_cast<float>(last - first), s);
     std::vector<double> h(lines.size());
-    check(hipMemcpyAsync(h.data(), lines.get(), lines.bytes(), hipMemcpyDeviceToHost, s), "download");
+    download(h.data(), lines.get(), lines.bytes(), s);
     check(hipStreamSynchronize(s), "bpm_gradient_dev");
     double sum = 0;
     for (int a = first; a < last; ++a) {
@@ -828,9 +898,9 @@ int qh_hip_tv_prox(const float* b, int nz, int ny, int nx, float lambda, int inn
     TvState state(nz, ny, nx);
     state.set_fused(fused != 0);
     state.reset(s);
-    check(hipMemcpyAsync(db.get(), b, db.bytes(), hipMemcpyHostToDevice, s), "upload");
+    upload(db.get(), b, db.bytes(), s);
     state.prox(db.get(), dx.get(), lambda, inner, s);
-    check(hipMemcpyAsync(x, dx.get(), dx.bytes(), hipMemcpyDeviceToHost, s), "download");
+    download(x, dx.get(), dx.bytes(), s);
     check(hipStreamSynchronize(s), "tv_prox");
   });
 }

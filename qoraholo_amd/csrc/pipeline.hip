@@ -5,6 +5,7 @@
 #include <chrono>
 
 #include "fft.hpp"
+#include "hostio.hpp"
 #include "pipeline.hpp"
 
 namespace qh {
@@ -17,6 +18,11 @@ namespace {
 double since_ms(std::chrono::steady_clock::time_point t0) {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
+
+// The volume leaves the device in this many parts; holograms from a pageable buffer enter in
+// pieces of this size.
+constexpr int kOutChunks = 4;
+constexpr std::size_t kInPiece = std::size_t(8) << 20;
 
 }  // namespace
 
@@ -39,6 +45,7 @@ Pipeline::Pipeline(const qh_pipeline_options& o)
       output_(static_cast<std::size_t>(solver_.nz()) * solver_.n() * solver_.n()),
       filled_(1),
       chunk_done_(chunks_),
+      out_done_(kOutChunks),
       clock_(kMarks) {
   std::vector<float> il(2 * static_cast<std::size_t>(count_));
   for (int a = 0; a < count_; ++a) {
@@ -70,27 +77,32 @@ void Pipeline::set_background(const float* host) {
 
 // Fields of the sample holograms, their ratio to the background fields and the Rytov field.
 // The 1/N^2 scale and the (-1)^(x+y) output sign cancel in the ratio and are skipped.
-void Pipeline::front_end(const float* holo_dev, hipStream_t s, bool from_host) {
+void Pipeline::front_end(const float* holo_dev, hipStream_t s, bool from_host, const float* user) {
   const std::size_t image = static_cast<std::size_t>(n_) * n_;
   const std::size_t field = static_cast<std::size_t>(m_) * m_;
   const int h = n_ / 2 + 1;
   clock_.mark(kStart, s);
-  if (from_host) {
-    const hipStream_t c = copy_.get();
-    check(hipStreamWaitEvent(c, clock_.event(kStart).get(), 0), "hipStreamWaitEvent");
-    for (int k = 0; k < chunks_; ++k) {
-      const int first = k * chunk_;
-      const int valid = std::min(chunk_, count_ - first);
-      check(hipMemcpyAsync(holo_.get() + first * image, input_.get() + first * image,
-                           valid * image * sizeof(float), hipMemcpyHostToDevice, c),
-            "upload holograms");
-      chunk_done_[k].record(c);
-    }
-  }
+  const hipStream_t c = copy_.get();
+  if (from_host) check(hipStreamWaitEvent(c, clock_.event(kStart).get(), 0), "hipStreamWaitEvent");
   for (int k = 0; k < chunks_; ++k) {
     const int first = k * chunk_;
     const int valid = std::min(chunk_, count_ - first);
-    if (from_host) check(hipStreamWaitEvent(s, chunk_done_[k].get(), 0), "hipStreamWaitEvent");
+    if (from_host) {
+      // From the caller's pageable buffer: pieces of the chunk are copied into the pinned input
+      // and queued for upload one after the other, so host copies and DMA overlap.
+      const std::size_t bytes = valid * image * sizeof(float);
+      const std::size_t piece = user ? kInPiece : bytes;
+      for (std::size_t a = 0; a < bytes; a += piece) {
+        const std::size_t len = std::min(piece, bytes - a);
+        char* pinned = reinterpret_cast<char*>(input_.get() + first * image) + a;
+        if (user) host_copy(pinned, reinterpret_cast<const char*>(user + first * image) + a, len);
+        check(hipMemcpyAsync(reinterpret_cast<char*>(holo_.get() + first * image) + a, pinned, len,
+                             hipMemcpyHostToDevice, c),
+              "upload holograms");
+      }
+      chunk_done_[k].record(c);
+      check(hipStreamWaitEvent(s, chunk_done_[k].get(), 0), "hipStreamWaitEvent");
+    }
     const float* src = (from_host ? holo_.get() : holo_dev) + first * image;
     float2* half = half_.get() + static_cast<std::size_t>(first) * n_ * h;
     hologram_spectra(src, valid, n_, half, s);
@@ -104,19 +116,40 @@ void Pipeline::front_end(const float* holo_dev, hipStream_t s, bool from_host) {
   clock_.mark(kRytov, s);
 }
 
-void Pipeline::process() {
+void Pipeline::process() { process_host(nullptr, nullptr); }
+
+void Pipeline::run(const float* holograms, float* ri) { process_host(holograms, ri); }
+
+void Pipeline::process_host(const float* user_in, float* user_out) {
   if (!has_background_) throw LogicError("Pipeline: set_background() first");
   const auto t0 = std::chrono::steady_clock::now();
   const hipStream_t s = compute_.get();
-  front_end(nullptr, s, true);
+  front_end(nullptr, s, true, user_in);
   solver_.run(fields_.get(), s, &clock_, kSolver);
   const std::size_t vol = static_cast<std::size_t>(solver_.nz()) * solver_.n() * solver_.n();
-  check(hipMemcpyAsync(output_.get(), solver_.index(), vol * sizeof(float), hipMemcpyDeviceToHost, s),
-        "download index");
+  const std::size_t part = (vol + kOutChunks - 1) / kOutChunks;
+  for (int k = 0; k < kOutChunks; ++k) {
+    const std::size_t a = std::min(vol, k * part);
+    const std::size_t len = std::min(vol, a + part) - a;
+    if (len) {
+      check(hipMemcpyAsync(output_.get() + a, solver_.index() + a, len * sizeof(float),
+                           hipMemcpyDeviceToHost, s),
+            "download index");
+    }
+    out_done_[k].record(s);
+  }
   check(hipMemcpyAsync(filled_.get(), solver_.filled_device(), sizeof(unsigned long long),
                        hipMemcpyDeviceToHost, s),
         "download filled");
   clock_.mark(kDownloaded, s);
+  if (user_out) {
+    for (int k = 0; k < kOutChunks; ++k) {
+      const std::size_t a = std::min(vol, k * part);
+      const std::size_t len = std::min(vol, a + part) - a;
+      check(hipEventSynchronize(out_done_[k].get()), "hipEventSynchronize");
+      if (len) host_copy(user_out + a, output_.get() + a, len * sizeof(float));
+    }
+  }
   check(hipStreamSynchronize(s), "process");
   collect(true);
   timings_pending_ = false;

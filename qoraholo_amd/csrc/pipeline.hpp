@@ -21,6 +21,11 @@ class Pipeline {
   float* input() { return input_.get(); }
   const float* output() const { return output_.get(); }
   void process();
+  // process() from and to pageable host memory: each chunk of holograms is copied into the
+  // pinned input (on the copy threads) just before its upload is queued, so the copies overlap
+  // the uploads and transforms of the chunks before it; the volume leaves in chunks whose host
+  // copies overlap the DMA of the next chunk.
+  void run(const float* holograms, float* ri);
   void process_device(const float* holo_dev, float* ri_dev, hipStream_t s);
   double filled() const;
   // Stage timings of the last call; after process_device they are read once its work is done.
@@ -38,7 +43,8 @@ class Pipeline {
 
  private:
   enum { kStart = 0, kFields, kRytov, kSolver, kDownloaded = kSolver + kOdtMarks, kMarks };
-  void front_end(const float* holo_dev, hipStream_t s, bool from_host);
+  void front_end(const float* holo_dev, hipStream_t s, bool from_host, const float* user = nullptr);
+  void process_host(const float* user_in, float* user_out);
   void collect(bool downloaded);
 
   int count_, n_, m_, chunk_, chunks_;
@@ -57,6 +63,7 @@ class Pipeline {
   PinnedBuffer<float> output_;
   PinnedBuffer<unsigned long long> filled_;
   std::vector<Event> chunk_done_;
+  std::vector<Event> out_done_;
   StageClock clock_;
   qh_pipeline_timings timings_{};
   double setup_ms_ = 0;

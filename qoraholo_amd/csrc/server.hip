@@ -1,6 +1,5 @@
 // Served reconstruction loop (see server.hpp).
 #include <algorithm>
-#include <cstring>
 
 #include "server.hpp"
 
@@ -192,18 +191,12 @@ void Server::slot_loop(Slot* slot) {
     slot->cv.wait(lock, [&] { return slot->stop || slot->work; });
     if (slot->stop && !slot->work) return;
     std::shared_ptr<Request> r = slot->work;
-    const Geometry& g = *geometries_[r->geometry];
-    const std::size_t in_count = static_cast<std::size_t>(g.options.count) *
-                                 g.options.hologram_size * g.options.hologram_size;
-    const std::size_t out_count = static_cast<std::size_t>(g.nz) * g.n * g.n;
     lock.unlock();
     int status = QH_OK;
     std::string error;
     double filled = 0;
     try {
-      std::memcpy(slot->pipeline->input(), r->holograms, in_count * sizeof(float));
-      slot->pipeline->process();
-      std::memcpy(r->out, slot->pipeline->output(), out_count * sizeof(float));
+      slot->pipeline->run(r->holograms, r->out);
       filled = slot->pipeline->filled();
     } catch (const InvalidArgument& e) {
       status = QH_ERR_INVALID_ARGUMENT;

@@ -168,3 +168,65 @@ def test_bpm_argument_checks(qh):
         qh.bpm_reconstruct(fields[:2], il, g, ri)
     with pytest.raises(ValueError):
         qh.bpm_vjp(ri - np.float32(1.337), fields[:2], il, g)
+
+
+def test_host_copy_threads_any_size_and_alignment():
+    """qh_host_copy splits large copies over the library's copy threads: every size, offset
+    and concurrent caller gets exactly the source bytes."""
+    import threading
+
+    from qoraholo_amd import _native
+
+    rng = np.random.default_rng(11)
+    src = rng.integers(0, 256, (48 << 20) + 777, dtype=np.uint8)
+    for size, off in [(0, 0), (1, 3), (4096, 0), ((2 << 20) + 5, 1), ((3 << 20) - 1, 7),
+                      ((33 << 20) + 123, 13)]:
+        dst = np.zeros(size + 64, np.uint8)
+        _native.check(_native.lib().qh_host_copy(dst.ctypes.data + off, src.ctypes.data + off,
+                                                 size))
+        assert np.array_equal(dst[off:off + size], src[off:off + size])
+        assert not dst[:off].any() and not dst[off + size:].any()
+
+    outs = [np.zeros(len(src), np.uint8) for _ in range(6)]
+    errors = []
+
+    def copy(i):
+        try:
+            for _ in range(3):
+                _native.check(_native.lib().qh_host_copy(outs[i].ctypes.data, src.ctypes.data,
+                                                         len(src)))
+        except Exception as e:  # pragma: no cover - reported below
+            errors.append(e)
+
+    threads = [threading.Thread(target=copy, args=(i,)) for i in range(len(outs))]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errors
+    for o in outs:
+        assert np.array_equal(o, src)
+
+
+@pytest.mark.parametrize("count,bcount,shape", [(49, 1, (64, 64)), (7, 7, (33, 40)),
+                                                 (3, 1, (1, 5)), (40, 40, (128, 128))])
+def test_field_ratio_matches_oracle_bitwise(qh, oracle, count, bcount, shape):
+    """The host field_ratio (src/cpu/field.cpp:62-74) on the library's threads equals the
+    oracle's std::complex<float> loop bit for bit, for one background or one per field,
+    including zeros, infinities and tiny denominators."""
+    rng = np.random.default_rng(count)
+    s = (rng.standard_normal((count,) + shape) + 1j * rng.standard_normal((count,) + shape))
+    b = (rng.standard_normal((bcount,) + shape) + 1j * rng.standard_normal((bcount,) + shape))
+    s = s.astype(np.complex64)
+    b = b.astype(np.complex64)
+    b.flat[0] = 0
+    b.flat[-1] = 1e-30 + 1e-30j
+    b.flat[2] = complex(np.inf, 1.0)
+    b.flat[3] = 3e38 + 3e38j
+    s.flat[1] = np.inf
+    s.flat[4] = complex(np.nan, 2.0)
+    s.flat[5] = 3e38 - 3e38j
+    got = qh.field_ratio(s, b)
+    want = oracle.field_ratio(s, b)
+    assert got.shape == want.shape
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
