@@ -235,6 +235,11 @@ size_t qh_hip_pipeline_device_bytes(const qh_pipeline* p);
 int qh_hip_pipeline_output_shape(const qh_pipeline* p, int* nz, int* ny, int* nx);
 /* Records FGP kernel intervals in timings (a few events per outer iteration). */
 int qh_hip_pipeline_time_fgp(qh_pipeline* p, int on);
+/* No counterpart in the reference: 1 (default) replays the device work of a request (field
+ * extraction through the index volume, stage events included) as one captured hipGraph, built
+ * on the first request and again when the device input address or FGP timing changes; 0
+ * launches it kernel by kernel. Results are the same bit for bit. */
+int qh_hip_pipeline_use_graph(qh_pipeline* p, int on);
 
 /* ---- Served loop (no counterpart in the reference, which serves one request per Pipeline
  * call): requests of several geometries run concurrently in device-resident slots admitted

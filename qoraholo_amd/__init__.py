@@ -461,6 +461,11 @@ class Pipeline:
         """Record the FGP kernel intervals in ``timings`` (fgp_ms, fgp_steps)."""
         _native.check(_native.lib().qh_hip_pipeline_time_fgp(self._handle, int(bool(on))))
 
+    def use_graph(self, on: bool = True) -> None:
+        """Replay each request's device work as one captured hipGraph (the default) or launch
+        it kernel by kernel; the results are the same bit for bit."""
+        _native.check(_native.lib().qh_hip_pipeline_use_graph(self._handle, int(bool(on))))
+
     @property
     def filled(self) -> float:
         """Fraction of k-space voxels that received data in the last run."""

@@ -107,6 +107,7 @@ _SIGNATURES = {
     "qh_hip_pipeline_output": (c_void_p, [c_void_p]),
     "qh_hip_pipeline_process": (c_int, [c_void_p]),
     "qh_hip_pipeline_run": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "qh_hip_pipeline_use_graph": (c_int, [c_void_p, c_int]),
     "qh_hip_pipeline_process_device": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "qh_hip_pipeline_filled": (c_double, [c_void_p]),
     "qh_hip_pipeline_timings": (c_int, [c_void_p, POINTER(PipelineTimings_t)]),

@@ -585,6 +585,13 @@ int qh_hip_pipeline_time_fgp(qh_pipeline* p, int on) {
   });
 }
 
+int qh_hip_pipeline_use_graph(qh_pipeline* p, int on) {
+  return guarded([&] {
+    require(p != nullptr, "Pipeline: null handle");
+    p->impl->use_graph(on != 0);
+  });
+}
+
 int qh_hip_server_create(size_t budget_bytes, int max_slots, qh_server** out) {
   return guarded([&] {
     require(out != nullptr, "Server: null argument");

@@ -81,7 +81,7 @@ void Pipeline::front_end(const float* holo_dev, hipStream_t s, bool from_host, c
   const std::size_t image = static_cast<std::size_t>(n_) * n_;
   const std::size_t field = static_cast<std::size_t>(m_) * m_;
   const int h = n_ / 2 + 1;
-  clock_.mark(kStart, s);
+  if (!capturing_) clock_.mark(kStart, s);
   const hipStream_t c = copy_.get();
   if (from_host) check(hipStreamWaitEvent(c, clock_.event(kStart).get(), 0), "hipStreamWaitEvent");
   for (int k = 0; k < chunks_; ++k) {
@@ -109,11 +109,11 @@ void Pipeline::front_end(const float* holo_dev, hipStream_t s, bool from_host, c
     crop_fields(half, valid, n_, crop_, fields_.get() + first * field, s);
   }
   fields_inverse(fields_.get(), count_, m_, 1.0f, false, s);
-  clock_.mark(kFields, s);
+  if (!capturing_) clock_.mark(kFields, s);
   ratio_phase(fields_.get(), background_.get(), static_cast<long>(count_) * field, psi_.get(),
               logamp_.get(), s);
   unwrapper_.rytov(psi_.get(), logamp_.get(), fields_.get(), s);
-  clock_.mark(kRytov, s);
+  if (!capturing_) clock_.mark(kRytov, s);
 }
 
 void Pipeline::process() { process_host(nullptr, nullptr); }
@@ -124,8 +124,31 @@ void Pipeline::process_host(const float* user_in, float* user_out) {
   if (!has_background_) throw LogicError("Pipeline: set_background() first");
   const auto t0 = std::chrono::steady_clock::now();
   const hipStream_t s = compute_.get();
-  front_end(nullptr, s, true, user_in);
-  solver_.run(fields_.get(), s, &clock_, kSolver);
+  if (use_graph_ && !solver_.timing_fgp()) {
+    // Uploads into holo_ on the copy stream once the previous request's work on s is done,
+    // then the captured front end and solver, which read holo_.
+    const hipStream_t c = copy_.get();
+    const std::size_t image = static_cast<std::size_t>(n_) * n_;
+    upload_start_.record(s);
+    check(hipStreamWaitEvent(c, upload_start_.get(), 0), "hipStreamWaitEvent");
+    const std::size_t bytes = count_ * image * sizeof(float);
+    const std::size_t piece = user_in ? kInPiece : bytes;
+    for (std::size_t a = 0; a < bytes; a += piece) {
+      const std::size_t len = std::min(piece, bytes - a);
+      char* pinned = reinterpret_cast<char*>(input_.get()) + a;
+      if (user_in) host_copy(pinned, reinterpret_cast<const char*>(user_in) + a, len);
+      check(hipMemcpyAsync(reinterpret_cast<char*>(holo_.get()) + a, pinned, len,
+                           hipMemcpyHostToDevice, c),
+            "upload holograms");
+    }
+    chunk_done_.back().record(c);
+    check(hipStreamWaitEvent(s, chunk_done_.back().get(), 0), "hipStreamWaitEvent");
+    compute(holo_.get(), s);
+  } else {
+    graph_ran_ = false;
+    front_end(nullptr, s, true, user_in);
+    solver_.run(fields_.get(), s, &clock_, kSolver);
+  }
   const std::size_t vol = static_cast<std::size_t>(solver_.nz()) * solver_.n() * solver_.n();
   const std::size_t part = (vol + kOutChunks - 1) / kOutChunks;
   for (int k = 0; k < kOutChunks; ++k) {
@@ -153,14 +176,58 @@ void Pipeline::process_host(const float* user_in, float* user_out) {
   check(hipStreamSynchronize(s), "process");
   collect(true);
   timings_pending_ = false;
-  timings_.upload = elapsed_ms(clock_.event(kStart), chunk_done_.back());
+  timings_.upload = graph_ran_ ? elapsed_ms(upload_start_, chunk_done_.back())
+                               : elapsed_ms(clock_.event(kStart), chunk_done_.back());
   timings_.total = since_ms(t0);
+}
+
+void Pipeline::compute(const float* holo_dev, hipStream_t s) {
+  graph_ran_ = use_graph_ && !solver_.timing_fgp();
+  if (!graph_ran_) {
+    front_end(holo_dev, s, false);
+    solver_.run(fields_.get(), s, &clock_, kSolver);
+    return;
+  }
+  if (!graph_.ready() || graph_input_ != holo_dev) {
+    // Captured on the pipeline's own stream (a caller's stream may be the null stream, which
+    // cannot be captured), without the stage marks: events recorded inside a capture only
+    // order work within the graph and are not time-stamped by its replays.
+    // A capture can be invalidated by work other threads start meanwhile (allocations and
+    // synchronous copies on the null stream); the pipeline then stays with kernel-by-kernel
+    // launches, whose results are the same.
+    const hipStream_t cs = compute_.get();
+    check(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    capturing_ = true;
+    try {
+      front_end(holo_dev, cs, false);
+      solver_.run(fields_.get(), cs, nullptr, 0);
+      capturing_ = false;
+      graph_.end_capture(cs);
+      graph_input_ = holo_dev;
+    } catch (const DeviceError&) {
+      capturing_ = false;
+      GraphExec::abandon_capture(cs);
+      graph_.reset();
+      use_graph_ = false;
+      graph_ran_ = false;
+      front_end(holo_dev, s, false);
+      solver_.run(fields_.get(), s, &clock_, kSolver);
+      return;
+    } catch (...) {
+      capturing_ = false;
+      GraphExec::abandon_capture(cs);
+      graph_.reset();
+      throw;
+    }
+  }
+  clock_.mark(kStart, s);
+  graph_.launch(s);
+  clock_.mark(kSolver + kFinish, s);
 }
 
 void Pipeline::process_device(const float* holo_dev, float* ri_dev, hipStream_t s) {
   if (!has_background_) throw LogicError("Pipeline: set_background() first");
-  front_end(holo_dev, s, false);
-  solver_.run(fields_.get(), s, &clock_, kSolver);
+  compute(holo_dev, s);
   const std::size_t vol = static_cast<std::size_t>(solver_.nz()) * solver_.n() * solver_.n();
   check(hipMemcpyAsync(ri_dev, solver_.index(), vol * sizeof(float), hipMemcpyDeviceToDevice, s),
         "copy index");
@@ -184,6 +251,16 @@ const qh_pipeline_timings& Pipeline::timings() {
 
 void Pipeline::collect(bool downloaded) {
   const int o = kSolver;
+  if (graph_ran_) {
+    timings_.fields = timings_.rytov = timings_.fft2 = timings_.clear = timings_.mapping = 0;
+    timings_.normalize = timings_.finish = 0;
+    timings_.solve = clock_.ms(kStart, o + kFinish);
+    timings_.download = downloaded ? clock_.ms(o + kFinish, kDownloaded) : 0.0;
+    timings_.fgp_ms = 0;
+    timings_.fgp_steps = 0;
+    timings_.fgp_launches = 0;
+    return;
+  }
   timings_.fields = clock_.ms(kStart, kFields);
   timings_.rytov = clock_.ms(kFields, kRytov);
   timings_.fft2 = clock_.ms(kRytov, o + kFft2);

@@ -35,6 +35,14 @@ class Pipeline {
   int nz() const { return solver_.nz(); }
   int n() const { return solver_.n(); }
   void time_fgp(bool on) { solver_.time_fgp(on); }
+  // Replay the device work of a request (front end and solver) as one captured graph (default)
+  // or launch it kernel by kernel. Graph replays time the whole device work only (reported as
+  // the solve stage); stage splits and FGP intervals come from kernel-by-kernel runs, which
+  // time_fgp(true) selects.
+  void use_graph(bool on) {
+    use_graph_ = on;
+    if (!on) graph_.reset();
+  }
   bool has_background() const { return has_background_; }
   void set_setup(double ms, std::size_t bytes) {
     setup_ms_ = ms;
@@ -45,6 +53,9 @@ class Pipeline {
   enum { kStart = 0, kFields, kRytov, kSolver, kDownloaded = kSolver + kOdtMarks, kMarks };
   void front_end(const float* holo_dev, hipStream_t s, bool from_host, const float* user = nullptr);
   void process_host(const float* user_in, float* user_out);
+  // Queues the front end and the solver on s reading holograms at holo_dev: through the graph
+  // captured for holo_dev (captured now if there is none), or kernel by kernel.
+  void compute(const float* holo_dev, hipStream_t s);
   void collect(bool downloaded);
 
   int count_, n_, m_, chunk_, chunks_;
@@ -64,6 +75,12 @@ class Pipeline {
   PinnedBuffer<unsigned long long> filled_;
   std::vector<Event> chunk_done_;
   std::vector<Event> out_done_;
+  Event upload_start_;
+  GraphExec graph_;
+  const float* graph_input_ = nullptr;
+  bool use_graph_ = true;
+  bool graph_ran_ = false;  // the last request ran through the graph
+  bool capturing_ = false;  // stage marks are skipped while the graph is captured
   StageClock clock_;
   qh_pipeline_timings timings_{};
   double setup_ms_ = 0;

@@ -134,6 +134,41 @@ class Event {
   hipEvent_t e_ = nullptr;
 };
 
+// An instantiated graph: the kernels, copies and event records captured from a stream, replayed
+// with one launch.
+class GraphExec {
+ public:
+  GraphExec() = default;
+  ~GraphExec() { reset(); }
+  GraphExec(const GraphExec&) = delete;
+  GraphExec& operator=(const GraphExec&) = delete;
+  void reset() {
+    if (g_) (void)hipGraphExecDestroy(g_);
+    g_ = nullptr;
+  }
+  // Ends the capture begun on s and instantiates what it recorded.
+  void end_capture(hipStream_t s) {
+    reset();
+    hipGraph_t graph = nullptr;
+    check(hipStreamEndCapture(s, &graph), "hipStreamEndCapture");
+    const hipError_t e = hipGraphInstantiate(&g_, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    check(e, "hipGraphInstantiate");
+  }
+  // Ends a capture that failed part way and drops what it recorded.
+  static void abandon_capture(hipStream_t s) {
+    hipGraph_t graph = nullptr;
+    (void)hipStreamEndCapture(s, &graph);
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipGetLastError();
+  }
+  void launch(hipStream_t s) const { check(hipGraphLaunch(g_, s), "hipGraphLaunch"); }
+  bool ready() const { return g_ != nullptr; }
+
+ private:
+  hipGraphExec_t g_ = nullptr;
+};
+
 inline double elapsed_ms(const Event& a, const Event& b) {
   float ms = 0;
   check(hipEventElapsedTime(&ms, a.get(), b.get()), "hipEventElapsedTime");

@@ -63,6 +63,10 @@ int Server::add_geometry(const qh_pipeline_options& o, const float* backgrounds)
   const std::size_t before = free_device_bytes();
   auto slot = std::make_unique<Slot>();
   slot->pipeline = std::make_unique<Pipeline>(g->options);
+  // Slots launch kernel by kernel: a graph capture on one slot's thread would be invalidated
+  // by another slot being set up (allocations, synchronous copies) meanwhile, and with several
+  // slots in flight the launch gaps are covered by the other slots' work.
+  slot->pipeline->use_graph(false);
   slot->pipeline->set_background(g->backgrounds.data());
   check(hipDeviceSynchronize(), "Server::add_geometry");
   const std::size_t after = free_device_bytes();
@@ -144,6 +148,7 @@ bool Server::try_admit(const std::shared_ptr<Request>& r) {
   auto slot = std::make_unique<Slot>();
   slot->geometry = r->geometry;
   slot->pipeline = std::make_unique<Pipeline>(g.options);
+  slot->pipeline->use_graph(false);  // as in add_geometry
   slot->pipeline->set_background(g.backgrounds.data());
   slot->bytes = g.slot_bytes;
   used_bytes_ += slot->bytes;

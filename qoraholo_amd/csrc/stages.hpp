@@ -125,6 +125,7 @@ class OdtSolver {
   int fgp_steps() const { return fgp_steps_; }
   int fgp_launches() const { return fgp_launches_; }
   void time_fgp(bool on) { time_fgp_ = on; }
+  bool timing_fgp() const { return time_fgp_; }
 
  private:
   void map(float2* fields, hipStream_t s, StageClock* clock, int first, bool fold);

@@ -259,6 +259,67 @@ def test_pipeline_matches_oracle_and_reuses_buffers(device, oracle, qh, chunks):
     assert t["total"] > 0 and t["solve"] > 0
 
 
+@pytest.mark.parametrize("solver", ["gp", "tv"])
+def test_pipeline_graph_replays_equal_kernel_launches(device, qh, solver):
+    """The captured graph of a request (front end and solver) gives the bits of launching the
+    same work kernel by kernel, on every replay, from host arrays and from device memory, with
+    other GPU work and synchronisations between the replays, after a change of input address
+    (a new capture) and with the FGP timing events inside the graph."""
+    torch = pytest.importorskip("torch")
+    d = PipelineData(qh, iterations=4, solver=solver)
+    p = qh.Pipeline(d.count, d.size, illuminations=d.illum, geometry=d.geom, **d.kw)
+    p.set_background(d.bg)
+    print(f"pipeline graph {solver}", flush=True)
+    p.use_graph(False)
+    eager = p.run(d.holo)
+    eager_filled = p.filled
+    holo_t = torch.from_numpy(d.holo).to("cuda:0")
+    out = torch.empty(p.output_shape, dtype=torch.float32, device="cuda:0")
+    p.run_device(holo_t, out)
+    torch.cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy(), eager)
+
+    p.use_graph(True)
+    crop = dict(center=(36.0, 36.0), radius=30.0, output_size=d.size // 2)
+    for _ in range(3):
+        assert np.array_equal(p.run(d.holo), eager)
+        assert p.filled == eager_filled
+        qh.extract_fields(d.holo, **crop)  # other GPU work between replays
+        torch.cuda.synchronize()
+    for _ in range(3):
+        out.zero_()
+        p.run_device(holo_t, out)
+        torch.cuda.synchronize()
+        assert np.array_equal(out.cpu().numpy(), eager)
+        torch.cuda.synchronize()
+    moved = holo_t.clone()
+    p.run_device(moved, out)
+    torch.cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy(), eager)
+
+    # A pipeline that never launched kernel by kernel: graph replays time the whole device
+    # work (as the solve stage); FGP timing switches to kernel-by-kernel launches.
+    q = qh.Pipeline(d.count, d.size, illuminations=d.illum, geometry=d.geom, **d.kw)
+    q.set_background(d.bg)
+    for _ in range(2):
+        q.run_device(holo_t, out)
+        torch.cuda.synchronize()
+        assert np.array_equal(out.cpu().numpy(), eager)
+        t = q.timings
+        assert t["solve"] > 0 and t["fields"] == 0 and t["total"] >= t["solve"]
+    got = q.run(d.holo)
+    assert np.array_equal(got, eager)
+    t = q.timings
+    assert t["upload"] > 0 and t["solve"] > 0 and t["download"] > 0
+    if solver == "tv":
+        q.time_fgp(True)
+        q.run_device(holo_t, out)
+        torch.cuda.synchronize()
+        assert np.array_equal(out.cpu().numpy(), eager)
+        t = q.timings
+        assert 0 < t["fgp_ms"] < t["solve"] and t["fgp_steps"] == 4 * 10 and t["fields"] > 0
+
+
 def test_pipeline_needs_background(device, qh):
     d = PipelineData(qh, angles=3)
     p = qh.Pipeline(d.count, d.size, illuminations=d.illum, geometry=d.geom, **d.kw)
