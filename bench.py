@@ -614,6 +614,22 @@ def main() -> int:
         raw = float(np.sqrt(np.sum((got - want) ** 2) / np.sum(want.astype(np.float64) ** 2)))
         result["parity"] = {"contrast_rel_l2_vs_oracle": err, "ri_rel_l2_vs_oracle": raw,
                             "tolerance": 5e-5}
+        # The same reconstruction stated against the oracle in double precision: the device and
+        # the float oracle (the reference's arithmetic) are about equally far from it.
+        import oracle
+
+        g = w.geometry
+        cy, cx = (int(np.sign(c) * np.floor(abs(c) + 0.5)) for c in w.center)
+        t0 = time.perf_counter()
+        want64, _ = oracle.pipeline(w.holograms, w.backgrounds, cy, cx, w.radius, N,
+                                    w.illuminations, (g.wavelength, g.n_medium, g.na, g.pixel),
+                                    dtype=np.float64, iterations=ITERATIONS, solver="tv",
+                                    tv_weight=TV_WEIGHT, tv_inner=INNER)
+        log(f"fp64 oracle: one reconstruction in {time.perf_counter() - t0:.2f} s")
+        c64 = np.sum((want64 - nm) ** 2)
+        result["parity"]["contrast_rel_l2_vs_fp64"] = float(np.sqrt(np.sum((got - want64) ** 2) / c64))
+        result["parity"]["oracle_float_contrast_rel_l2_vs_fp64"] = float(
+            np.sqrt(np.sum((want.astype(np.float64) - want64) ** 2) / c64))
     print(json.dumps(result), flush=True)
     return 0
 

@@ -477,3 +477,56 @@ def test_solver_table_reproduces_published_accuracy(device, qh, n):
         digits = 4 if rmse == 1.002e-3 else 3
         assert printed_close(got, rmse, digits), (kw, got, rmse)
         assert abs(float(v[mask].mean()) - mean) <= 0.6e-4, (kw, float(v[mask].mean()), mean)
+
+
+def test_every_stage_against_fp64(device, oracle, qh):
+    """Every intermediate of the chain on the bead workload (49 x 256^2 -> 128^3): fields,
+    their ratio, the unwrapped phase, the Rytov field, the k-space potential (counts bit-exact)
+    and the index after GP 20 and FISTA-TV 5x10, against the oracle in float (within 5e-5) and
+    in double: the device is as close to the FP64 statement as the float oracle (the reference's
+    arithmetic) is, within a factor 2, or within 2e-6."""
+    from qoraholo_amd import workloads
+
+    w = workloads.bead()
+    g = w.geometry
+    geom_t = (g.wavelength, g.n_medium, g.na, g.pixel)
+    cy, cx = (int(np.sign(c) * np.floor(abs(c) + 0.5)) for c in w.center)
+    crop = dict(center=w.center, radius=w.radius, output_size=128)
+
+    def check(name, got, f32, f64, sub=0.0):
+        e_gpu = rel_l2(got - sub, f64 - sub)
+        e_ref = rel_l2(f32 - sub, f64 - sub)
+        print(f"{name}: device vs fp64 {e_gpu:.2e}, float oracle vs fp64 {e_ref:.2e}", flush=True)
+        assert rel_l2(got - sub, f32 - sub) < TOL, name
+        assert e_gpu <= max(2 * e_ref, 2e-6), name
+
+    fs = qh.extract_fields(w.holograms, **crop)
+    fb = qh.extract_fields(w.backgrounds, **crop)
+    o32 = [oracle.extract_fields(h, cy, cx, w.radius, 128) for h in (w.holograms, w.backgrounds)]
+    o64 = [oracle.extract_fields(h, cy, cx, w.radius, 128, np.float64)
+           for h in (w.holograms, w.backgrounds)]
+    check("fields", fs, o32[0], o64[0])
+    ratio = qh.field_ratio(fs, fb)
+    r32 = oracle.field_ratio(o32[0], o32[1])
+    r64 = oracle.field_ratio(o64[0], o64[1], np.float64)
+    check("ratio", ratio, r32, r64)
+    psi = np.angle(r64).astype(np.float32)
+    check("unwrap", qh.unwrap(psi), oracle.unwrap(psi), oracle.unwrap(psi.astype(np.float64),
+                                                                       np.float64))
+    ry = qh.rytov(ratio)
+    y32 = oracle.rytov(r32)
+    y64 = oracle.rytov(r64, np.float64)
+    check("rytov", ry, y32, y64)
+    pot, counts, filled = qh.kspace(y64.astype(np.complex64), w.illuminations, g)
+    _, f32, k32, w32 = oracle.reconstruct(y64.astype(np.complex64), w.illuminations, geom_t,
+                                          intermediates=True)
+    _, f64, k64, w64 = oracle.reconstruct(y64, w.illuminations, geom_t, intermediates=True,
+                                          dtype=np.float64)
+    assert np.array_equal(counts.astype(np.float32), w32) and filled == f32
+    check("kspace", pot, k32, k64)
+    for kw in (dict(iterations=20), dict(iterations=5, solver="tv")):
+        got = qh.reconstruct(y64.astype(np.complex64), w.illuminations, g, **kw)
+        want32, _ = oracle.reconstruct(y64.astype(np.complex64), w.illuminations, geom_t, **kw)
+        want64, _ = oracle.reconstruct(y64, w.illuminations, geom_t, dtype=np.float64, **kw)
+        check(f"index {kw}", got.astype(np.float64), want32.astype(np.float64), want64,
+              sub=np.float64(g.n_medium))
