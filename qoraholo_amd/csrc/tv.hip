@@ -53,7 +53,7 @@ __device__ __forceinline__ void load_row(const float* base, const TvArgs& a, int
   const bool inside = z >= 0 && z < a.nz && y >= 0 && y < a.ny;
   if constexpr (VEC) {
     static_assert(VX % 2 == 0, "whole float2s");
-    if (inside && xb < a.nx) {
+    if (inside && xb >= 0 && xb < a.nx) {
       const float* p = base + (static_cast<long>(z) * a.ny + y) * a.nx + xb;
       if constexpr (VX % 4 == 0) {
 #pragma unroll
@@ -80,10 +80,10 @@ __device__ __forceinline__ void load_row(const float* base, const TvArgs& a, int
 #pragma unroll
     for (int v = 0; v < VX; ++v) out[v] = 0.0f;
     if (!inside) return;
-    const float* p = base + (static_cast<long>(z) * a.ny + y) * a.nx + xb;
+    const long row = (static_cast<long>(z) * a.ny + y) * a.nx;
 #pragma unroll
     for (int v = 0; v < VX; ++v) {
-      if (xb + v < a.nx) out[v] = p[v];
+      if (xb + v >= 0 && xb + v < a.nx) out[v] = base[row + xb + v];
     }
   }
 }
@@ -297,6 +297,8 @@ struct PairArgs {
   int first;
   float m2;  // momentum of r_{k+1} = p_{k+1} + m2 (p_{k+1} - p_k)
   int zlen;  // planes per chunk
+  int xt;       // output columns per x tile (nx when one tile spans the row)
+  int tiles_x;  // x tiles per row
 };
 
 // z = max(b - lambda L(r), 0) on one row of plane zp (the expression order of primal_plane).
@@ -316,8 +318,9 @@ __device__ __forceinline__ void pair_primal(const PairArgs& a, int zp, int y, in
     if (y > 0) s -= ry_up[v];
     if (x < a.nx - 1) s += R[2][v];
     if (x > 0) s -= (v == 0 ? left : R[2][v - 1]);
-    Z[v] = (y >= 0 && y < a.ny && x < a.nx && zp >= 0 && zp < a.nz) ? tv_primal(B[v], a.lambda, s)
-                                                                   : 0.0f;
+    Z[v] = (y >= 0 && y < a.ny && x >= 0 && x < a.nx && zp >= 0 && zp < a.nz)
+               ? tv_primal(B[v], a.lambda, s)
+               : 0.0f;
   }
 }
 
@@ -409,28 +412,36 @@ template <int VX, int NW, bool VEC>
 __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
   constexpr int TOP = 2;  // halo rows above the output rows (and two below)
   constexpr int T = NW - 4;
+  constexpr int LW = 64 * VX;  // columns one wave holds: the LDS row width
   extern __shared__ float4 lds4[];
   float* lds = reinterpret_cast<float*>(lds4);
-  const int nx = a.nx;
   float* ry = lds;              // y component of r_k of each row
-  float* rqy = ry + NW * nx;    // y component of r_{k+1}
-  float* zk = rqy + NW * nx;    // z_k
-  float* zq = zk + NW * nx;     // z_{k+1}
+  float* rqy = ry + NW * LW;    // y component of r_{k+1}
+  float* zk = rqy + NW * LW;    // z_k
+  float* zq = zk + NW * LW;     // z_{k+1}
   TvArgs bd{};
   bd.nz = a.nz;
   bd.ny = a.ny;
   bd.nx = a.nx;
   const int w = static_cast<int>(threadIdx.x) >> 6;
   const int lane = static_cast<int>(threadIdx.x) & 63;
-  // Tiles of one z chunk are consecutive, so the y neighbours that share halo rows run on the
-  // same XCD.
+  // Tiles of one z chunk are consecutive (x fastest, then y), so the neighbours that share halo
+  // rows and columns run on the same XCD. Rows wider than a wave are cut into x tiles of xt
+  // output columns; each tile's wave also holds one lane of columns on either side, which two
+  // steps of the 7-point stencil need, and stores only its own columns.
   const int tile = xcd_tile();
-  const int tiles_y = static_cast<int>(gridDim.x) / chunks;
-  const int y0 = (tile % tiles_y) * T;
-  const int z0 = (tile / tiles_y) * a.zlen;
+  const int tx = tile % a.tiles_x;
+  const int rest = tile / a.tiles_x;
+  const int tiles_y = static_cast<int>(gridDim.x) / (chunks * a.tiles_x);
+  const int y0 = (rest % tiles_y) * T;
+  const int z0 = (rest / tiles_y) * a.zlen;
   const int z1 = min(z0 + a.zlen, a.nz);
   const int y = y0 - TOP + w;
-  const int xb = lane * VX;
+  const int xo = tx * a.xt;                         // first output column of the tile
+  const int x0 = a.tiles_x > 1 ? xo - VX : 0;       // column of lane 0
+  const int xl = lane * VX;                         // column within the LDS row
+  const int xb = x0 + xl;
+  const bool out_col = xb >= xo && xb < min(xo + a.xt, a.nx);
   const int wu = w > 0 ? w - 1 : 0;       // the first and last waves read their own row: their
   const int wd = w < NW - 1 ? w + 1 : w;  // results only feed other halo values, never a store
   const bool out_row = w >= TOP && w < TOP + T && y < a.ny;
@@ -492,14 +503,14 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
   }
   float rk1[3][VX], pk1[3][VX], b1[VX], zk1[VX];
   load_step1(zs, rk1, pk1, b1);
-  lds_put<VX, VEC>(ry, w, nx, xb, rk1[1]);
+  lds_put<VX, VEC>(ry, w, LW, xl, rk1[1]);
   __syncthreads();
   {
     float up[VX];
-    lds_get<VX, VEC>(ry, wu, nx, xb, up);
+    lds_get<VX, VEC>(ry, wu, LW, xl, up);
     pair_primal<VX>(a, zs, y, xb, rk1, rz_m2, up, b1, zk1);
   }
-  lds_put<VX, VEC>(zk, w, nx, xb, zk1);
+  lds_put<VX, VEC>(zk, w, LW, xl, zk1);
   float rq0[3][VX], zq0[VX], zdn_k[VX], zdn_q[VX];
 #pragma unroll
   for (int v = 0; v < VX; ++v) {
@@ -509,7 +520,7 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
     for (int c = 0; c < 3; ++c) rq0[c][v] = 0.0f;
   }
   __syncthreads();
-  lds_get<VX, VEC>(zk, wd, nx, xb, zdn_k);  // z_k of the row below on plane zs
+  lds_get<VX, VEC>(zk, wd, LW, xl, zdn_k);  // z_k of the row below on plane zs
 
   // The loads of the next plane are issued before the current plane's last dual step, so that
   // their latency overlaps that work instead of stalling the block at the next barrier; r_k is
@@ -529,12 +540,12 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
     const int f = t + 2;
     float rk2[3][VX];
     extrapolate_k(pk2, pm2, rk2);
-    lds_put<VX, VEC>(ry, w, nx, xb, rk2[1]);
+    lds_put<VX, VEC>(ry, w, LW, xl, rk2[1]);
     __syncthreads();  // (1) r_k of plane f visible
     float zk2[VX] = {};
     if (need_zk) {
       float up[VX];
-      lds_get<VX, VEC>(ry, wu, nx, xb, up);
+      lds_get<VX, VEC>(ry, wu, LW, xl, up);
       pair_primal<VX>(a, f, y, xb, rk2, rk1[0], up, b2, zk2);
     }
     // p_{k+1} and r_{k+1} on plane t + 1.
@@ -542,7 +553,7 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
     if (need_pk1) {
       float pq1[3][VX];
       pair_dual<VX>(a, t + 1, y, xb, zk1, zk2, zdn_k, rk1, pq1);
-      if (a.p_k1 && out_row && t + 1 >= z0 && t + 1 < z1) {
+      if (a.p_k1 && out_row && out_col && t + 1 >= z0 && t + 1 < z1) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) store_row<VX, VEC>(a.p_k1 + c * a.vol, bd, t + 1, y, xb, pq1[c]);
       }
@@ -552,17 +563,17 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
         for (int v = 0; v < VX; ++v) rq1[c][v] = pq1[c][v] + a.m2 * (pq1[c][v] - pk1[c][v]);
       }
     }
-    lds_put<VX, VEC>(zk, w, nx, xb, zk2);
-    lds_put<VX, VEC>(rqy, w, nx, xb, rq1[1]);
+    lds_put<VX, VEC>(zk, w, LW, xl, zk2);
+    lds_put<VX, VEC>(rqy, w, LW, xl, rq1[1]);
     __syncthreads();  // (2) z_k of plane f and r_{k+1} of plane t + 1 visible
-    lds_get<VX, VEC>(zk, wd, nx, xb, zdn_k);  // for the next plane's dual step
+    lds_get<VX, VEC>(zk, wd, LW, xl, zdn_k);  // for the next plane's dual step
     float zq1[VX] = {};
     if (need_zq) {
       float up[VX];
-      lds_get<VX, VEC>(rqy, wu, nx, xb, up);
+      lds_get<VX, VEC>(rqy, wu, LW, xl, up);
       pair_primal<VX>(a, t + 1, y, xb, rq1, rq0[0], up, b1, zq1);
     }
-    lds_put<VX, VEC>(zq, w, nx, xb, zq1);
+    lds_put<VX, VEC>(zq, w, LW, xl, zq1);
     __syncthreads();  // (3) z_{k+1} of plane t + 1 visible
     float pk3[3][VX], pm3[3][VX], b3[VX];
     load_raw(f + 1, pk3, pm3, b3);
@@ -570,12 +581,12 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
     if (t >= z0 && need_pk2) {
       float p2[3][VX];
       pair_dual<VX>(a, t, y, xb, zq0, zq1, zdn_q, rq0, p2);
-      if (out_row) {
+      if (out_row && out_col) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) store_row<VX, VEC>(a.p_k2 + c * a.vol, bd, t, y, xb, p2[c]);
       }
     }
-    lds_get<VX, VEC>(zq, wd, nx, xb, zdn_q);  // z_{k+1} of the row below on plane t + 1
+    lds_get<VX, VEC>(zq, wd, LW, xl, zdn_q);  // z_{k+1} of the row below on plane t + 1
 #pragma unroll
     for (int v = 0; v < VX; ++v) {
       b1[v] = b2[v];
@@ -741,52 +752,55 @@ void tv_primal(const float* b, const float* p, int nz, int ny, int nx, float lam
 namespace {
 
 // Rows per block of the two-step kernel: NW waves, NW - 4 output rows, as many waves as the
-// registers of VX floats per lane leave room for in one block per CU (rows of 384 take 6 floats
-// per lane and 8 waves). Four row buffers of NW x nx floats stay within 64 KiB of LDS.
-int pair_waves(int nx) { return nx <= 128 ? 16 : (nx <= 256 ? 12 : 8); }
+// registers of VX floats per lane leave room for in one block per CU. Rows up to 256 wide are
+// held whole by a wave; wider rows are cut into x tiles of at most kPairTileX output columns
+// (one lane of 4 columns of halo on either side), so every width runs the 4-column variant.
+int pair_waves(int vx) { return vx == 1 ? 16 : (vx == 2 ? 16 : 12); }
+constexpr int kPairTileX = 64 * 4 - 2 * 4;
+
+template <int VX, int NW>
+void launch_pair(const PairArgs& a, int chunks, dim3 grid, hipStream_t s) {
+  // Four row buffers of NW x 64 VX floats: at most 48 KiB.
+  const std::size_t lds = 4 * static_cast<std::size_t>(NW) * 64 * VX * sizeof(float);
+  if constexpr (VX % 2 == 0) {
+    if (rows_vec<VX>(a.nx)) {
+      hipLaunchKernelGGL((k_fgp_pair<VX, NW, true>), grid, dim3(NW * 64), lds, s, a, chunks);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_fgp_pair<VX, NW, false>), grid, dim3(NW * 64), lds, s, a, chunks);
+}
 
 }  // namespace
 
-bool fgp_pair_supported(int nx) { return nx >= 1 && nx <= 384; }
+bool fgp_pair_supported(int nx) { return nx >= 1 && nx <= 512; }
 
 void fgp_pair(const float* b, const float* p_k, const float* p_km1, float* p_k1, float* p_k2,
               int nz, int ny, int nx, float lambda, float step, float m1, bool first, float m2,
               hipStream_t s) {
-  require(fgp_pair_supported(nx), "fgp_pair: rows up to 384 wide");
-  const int nw = pair_waves(nx);
+  require(fgp_pair_supported(nx), "fgp_pair: rows up to 512 wide");
+  const int vx = std::min(lanes_x(nx), 4);
+  int tiles_x = 1;
+  int xt = nx;
+  if (nx > 64 * 4) {
+    tiles_x = (nx + kPairTileX - 1) / kPairTileX;
+    xt = ((nx + tiles_x - 1) / tiles_x + 3) / 4 * 4;  // balanced tiles, whole lanes
+  }
+  const int nw = pair_waves(vx);
   const int rows = nw - 4;
   const int tiles_y = (ny + rows - 1) / rows;
   int zlen = kPairChunk;
-  while (zlen > kPairChunkMin && tiles_y * ((nz + zlen - 1) / zlen) < kPairTargetBlocks) zlen /= 2;
-  PairArgs a{b, p_k, p_km1, p_k1, p_k2, static_cast<long>(nz) * ny * nx, nz, ny, nx,
-             lambda, step, m1, first ? 1 : 0, m2, zlen};
+  while (zlen > kPairChunkMin && tiles_x * tiles_y * ((nz + zlen - 1) / zlen) < kPairTargetBlocks) {
+    zlen /= 2;
+  }
+  PairArgs a{b,  p_k,    p_km1, p_k1, p_k2,          static_cast<long>(nz) * ny * nx,
+             nz, ny,     nx,    lambda, step,        m1, first ? 1 : 0, m2, zlen, xt, tiles_x};
   const int chunks = (nz + zlen - 1) / zlen;
-  const dim3 grid(static_cast<unsigned>(tiles_y * chunks));
-  const dim3 block(static_cast<unsigned>(nw * 64));
-  const std::size_t lds = 4 * static_cast<std::size_t>(nw) * nx * sizeof(float);
-  switch (lanes_x(nx)) {
-    case 1: hipLaunchKernelGGL((k_fgp_pair<1, 16, false>), grid, block, lds, s, a, chunks); break;
-    case 2:
-      if (rows_vec<2>(nx)) {
-        hipLaunchKernelGGL((k_fgp_pair<2, 16, true>), grid, block, lds, s, a, chunks);
-      } else {
-        hipLaunchKernelGGL((k_fgp_pair<2, 16, false>), grid, block, lds, s, a, chunks);
-      }
-      break;
-    case 4:
-      if (rows_vec<4>(nx)) {
-        hipLaunchKernelGGL((k_fgp_pair<4, 12, true>), grid, block, lds, s, a, chunks);
-      } else {
-        hipLaunchKernelGGL((k_fgp_pair<4, 12, false>), grid, block, lds, s, a, chunks);
-      }
-      break;
-    default:
-      if (rows_vec<6>(nx)) {
-        hipLaunchKernelGGL((k_fgp_pair<6, 8, true>), grid, block, lds, s, a, chunks);
-      } else {
-        hipLaunchKernelGGL((k_fgp_pair<6, 8, false>), grid, block, lds, s, a, chunks);
-      }
-      break;
+  const dim3 grid(static_cast<unsigned>(tiles_x * tiles_y * chunks));
+  switch (vx) {
+    case 1: launch_pair<1, 16>(a, chunks, grid, s); break;
+    case 2: launch_pair<2, 16>(a, chunks, grid, s); break;
+    default: launch_pair<4, 12>(a, chunks, grid, s); break;
   }
   check_launch("fgp_pair");
 }
