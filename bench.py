@@ -287,16 +287,22 @@ def served_mixed(qh, rates=(0.25, 0.5, 1.0, 2.0), per_rate=16, seed=0):
     return out
 
 
-def served_bpm(qh, sizes=((192, 49), (384, 81)), iterations=40, seed=0):
-    """Config 5: BPM inversion requests from host arrays, one at a time. Each request is the
-    reference's recipe (examples/bpm_figure.py:74-88): Rytov of the field ratios, 20 GP
-    iterations for the start, then 40 full-batch FISTA iterations of the BPM with the TV prox
-    (weight 1e-2, 10 inner steps). Sizes 192^2 x 192 with 49 illuminations and 384^2 x 384 with
-    81; random cells at 0.2 um. Per iteration, the reference's traffic model of a gradient
-    (forward, 15/16 recomputation from checkpoints, adjoint) is the algorithmic figure."""
+def served_bpm(qh, sizes=((192, 49), (384, 81)), iterations=40, seed=0, repeats=(4, 2),
+               max_slots=4):
+    """Config 5: BPM inversion requests from host arrays. Each request is the reference's recipe
+    (examples/bpm_figure.py:74-88): Rytov of the field ratios, 20 GP iterations for the start,
+    then 40 full-batch FISTA iterations of the BPM with the TV prox (weight 1e-2, 10 inner
+    steps). Sizes 192^2 x 192 with 49 illuminations and 384^2 x 384 with 81; random cells at
+    0.2 um. First each size through the per-call functions, one at a time (the stage split and
+    the per-iteration time; per iteration, the reference's traffic model of a gradient --
+    forward, 15/16 recomputation from checkpoints, adjoint -- is the algorithmic figure); then
+    the same requests through the Server, its BPM geometries resident and admitted by their
+    measured HBM footprint: a burst of `repeats[k]` requests of size k, interleaved, with
+    inversions/s, p50/p99 latency and each result compared with the per-call one."""
     from qoraholo_amd import workloads
 
     out = []
+    reqs = []
     for n, a in sizes:
         t0 = time.perf_counter()
         req = workloads.bpm_request(n, a, seed=seed)
@@ -319,6 +325,7 @@ def served_bpm(qh, sizes=((192, 49), (384, 81)), iterations=40, seed=0):
         grad_alg = int((88 * (1 + 15 / 16) + 100) * n * n * a * n)
         truth = req.truth.astype(np.float64)
         row = {
+            "size": n,
             "workload": f"random cell {n}^2 x {n} at 0.2 um, {a} illuminations NA 0.9, "
                         f"Rytov + GP {ITERATIONS} start, {iterations} FISTA iterations, "
                         f"TV {TV_WEIGHT} x {INNER}",
@@ -338,7 +345,38 @@ def served_bpm(qh, sizes=((192, 49), (384, 81)), iterations=40, seed=0):
         }
         log(f"served bpm {n}^3 x {a}: {row}")
         out.append(row)
-    return {"requests": out}
+        reqs.append((req, ri, loss))
+
+    server = qh.Server(max_slots=max_slots)
+    gids = [server.add_bpm(n, req.illuminations, req.geometry, start_iterations=ITERATIONS,
+                           iterations=iterations, tv_weight=TV_WEIGHT, tv_inner=INNER)
+            for (n, _), (req, _, _) in zip(sizes, reqs)]
+    for k, gid in enumerate(gids):  # one untimed request per geometry
+        server.wait(server.submit_bpm(gid, reqs[k][0].fields))
+    server.reset_stats()
+    order = []
+    for i in range(max(repeats[:len(sizes)])):
+        order += [k for k in range(len(sizes)) if i < repeats[k]]
+    log(f"served bpm stream: {len(order)} requests, sizes {[sizes[k][0] for k in order]}")
+    t0 = time.perf_counter()
+    ids = [(k, server.submit_bpm(gids[k], reqs[k][0].fields)) for k in order]
+    same = True
+    for k, rid in ids:
+        ri, loss, _ = server.wait(rid)
+        same = same and bool(np.array_equal(ri, reqs[k][1])) and bool(np.array_equal(loss, reqs[k][2]))
+    wall = time.perf_counter() - t0
+    st = server.stats()
+    plain_s = sum(out[k]["request_ms"] for k in order) / 1000.0
+    stream = {"requests": len(order), "sizes": [sizes[k][0] for k in order],
+              "inversions_per_s": round(st["recon_per_s"], 4),
+              "wall_s": round(wall, 2),
+              "p50_ms": round(st["p50_ms"], 1), "p99_ms": round(st["p99_ms"], 1),
+              "per_call_inversions_per_s": round(len(order) / plain_s, 4),
+              "slots": st["slots"], "device_bytes": st["device_bytes"],
+              "budget_bytes": st["budget_bytes"],
+              "results_equal_per_call_bitwise": same}
+    log(f"served bpm stream: {stream}")
+    return {"requests": out, "served_stream": stream}
 
 
 def pct(v, q):

@@ -303,6 +303,33 @@ typedef struct {
   int tv_inner;
 } qh_bpm_solve_options;
 
+/* A BPM inversion request kind of the served loop (no counterpart in the reference, whose
+ * examples/bpm_figure.py:74-88 runs the same recipe call by call): Rytov fields of the field
+ * ratios, a Gerchberg-Papoulis start of start_iterations on the (nz, n, n) grid, then
+ * bpm_reconstruct with `solve` from that start. Results equal that chain of per-call functions
+ * (qh_hip_rytov, qh_hip_reconstruct, qh_hip_bpm_reconstruct with the contrast conversions of
+ * src/api.cpp:159-163) bit for bit. */
+typedef struct {
+  int n;
+  int nz;
+  int count;
+  const qh_illumination* illuminations;
+  qh_geometry geometry;
+  qh_bpm_options bpm;
+  int start_iterations;
+  qh_bpm_solve_options solve;
+} qh_bpm_request_options;
+
+/* Registers a BPM inversion geometry with the server; its first slot is built (and its device
+ * footprint measured) before the call returns. */
+int qh_hip_server_add_bpm(qh_server* s, const qh_bpm_request_options* options, int* geometry);
+/* Queues field ratios (count, n, n) of a BPM geometry; the refractive index (nz, n, n) goes to
+ * ri_out and, when loss is not NULL, the data term of each iteration to loss (solve.iterations
+ * values). Host buffers must stay valid until qh_hip_server_wait returns for the id. The same
+ * wait, poll and stats calls serve both request kinds. Thread-safe. */
+int qh_hip_server_submit_bpm(qh_server* s, int geometry, const qh_c64* fields, float* ri_out,
+                             double* loss, int64_t* id);
+
 /* The tables in float as bpm_tables (src/bpm.cpp:10-60) stores them: step, last (count, n, n)
  * complex, q (count, n, n), the screen phase k0 pixel and the Lipschitz constant. Host only. */
 int qh_bpm_tables(int n, int nz, const qh_illumination* illum, int count,

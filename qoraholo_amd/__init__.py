@@ -488,14 +488,16 @@ class Pipeline:
 
 
 class Server:
-    """Served reconstructions: requests of several geometries admitted against an HBM budget.
+    """Served requests of several geometries admitted against an HBM budget.
 
     ``add_geometry`` registers the options of a Pipeline with its background holograms and
     returns a geometry id; ``submit`` queues sample holograms of a geometry (thread-safe) and
-    returns a request id; ``wait`` returns (ri, filled, latency_ms). Requests that arrive while
-    others run are admitted into free device-resident slots (created while ``budget_bytes``
-    allows, at most ``max_slots`` in flight) and overlap on the device; each result is the one
-    the request gets alone.
+    returns a request id; ``wait`` returns (ri, filled, latency_ms). ``add_bpm`` registers a
+    BPM inversion geometry (Rytov of the field ratios, a Gerchberg-Papoulis start, then
+    ``bpm_reconstruct``), ``submit_bpm`` queues field ratios of one and ``wait`` returns
+    (ri, loss, latency_ms) for it. Requests that arrive while others run are admitted into free
+    device-resident slots (created while ``budget_bytes`` allows, at most ``max_slots`` in
+    flight) and overlap on the device; each result is the one the request gets alone.
     """
 
     def __init__(self, budget_bytes: int = 0, max_slots: int = 4):
@@ -541,13 +543,41 @@ class Server:
                                                                _native.ptr(b), ctypes.byref(gid)))
         m = o.crop.out
         self._geometries[gid.value] = ((count, hologram_size, hologram_size),
-                                       (nz if nz > 0 else m, m, m))
+                                       (nz if nz > 0 else m, m, m), None)
+        return gid.value
+
+    def add_bpm(self, n: int, illuminations, geometry: Geometry, nz: int = 0,
+                start_iterations: int = 20, iterations: int = 40, batch: int = 0,
+                step: float = 1.0, tv_weight: float = 0.0, tv_inner: int = 10,
+                obliquity: bool = True, chunk: int = 0, checkpoint: int = 16) -> int:
+        """Registers a BPM inversion geometry: field ratios (count, n, n) to a refractive index
+        (nz, n, n) by ``bpm_reconstruct`` with these options, started from the
+        ``start_iterations`` Gerchberg-Papoulis reconstruction of their Rytov fields."""
+        il = _illum(illuminations)
+        count = il.shape[0]
+        nzv = nz if nz > 0 else n
+        illum_c = (_native.Illumination_t * max(1, count))()
+        for a in range(count):
+            illum_c[a] = _native.Illumination_t(float(il[a, 0]), float(il[a, 1]))
+        o = _native.BpmRequestOptions_t(
+            int(n), int(nzv), int(count),
+            ctypes.cast(illum_c, ctypes.POINTER(_native.Illumination_t)), geometry._c(),
+            _native.BpmOptions_t(int(bool(obliquity)), int(chunk), int(checkpoint)),
+            int(start_iterations),
+            _native.BpmSolveOptions_t(int(iterations), int(batch), float(step), float(tv_weight),
+                                      int(tv_inner)))
+        gid = ctypes.c_int()
+        _native.check(_native.lib().qh_hip_server_add_bpm(self._handle, ctypes.byref(o),
+                                                          ctypes.byref(gid)))
+        self._geometries[gid.value] = ((count, n, n), (nzv, n, n), int(iterations))
         return gid.value
 
     def submit(self, geometry: int, holograms) -> int:
         if geometry not in self._geometries:
             raise ValueError("Server::submit: unknown geometry")
-        shape_in, shape_out = self._geometries[geometry]
+        shape_in, shape_out, bpm_iterations = self._geometries[geometry]
+        if bpm_iterations is not None:
+            raise ValueError("Server::submit: a BPM geometry takes submit_bpm")
         h = np.ascontiguousarray(holograms, dtype=np.float32)
         if h.shape != shape_in:
             raise ValueError(f"Server::submit: holograms must be {shape_in}")
@@ -556,7 +586,26 @@ class Server:
         _native.check(_native.lib().qh_hip_server_submit(self._handle, int(geometry),
                                                          _native.ptr(h), _native.ptr(out),
                                                          ctypes.byref(rid)))
-        self._pending[rid.value] = (h, out)
+        self._pending[rid.value] = (h, out, None)
+        return rid.value
+
+    def submit_bpm(self, geometry: int, fields) -> int:
+        """Queues field ratios (count, n, n) of a BPM geometry; returns the request id."""
+        if geometry not in self._geometries:
+            raise ValueError("Server::submit: unknown geometry")
+        shape_in, shape_out, iterations = self._geometries[geometry]
+        if iterations is None:
+            raise ValueError("Server::submit_bpm: not a BPM geometry")
+        f = np.ascontiguousarray(fields, dtype=np.complex64)
+        if f.shape != shape_in:
+            raise ValueError(f"Server::submit_bpm: fields must be {shape_in}")
+        out = np.empty(shape_out, np.float32)
+        loss = np.zeros(max(iterations, 1), np.float64)
+        rid = ctypes.c_int64()
+        _native.check(_native.lib().qh_hip_server_submit_bpm(
+            self._handle, int(geometry), _native.ptr(f), _native.ptr(out), _native.ptr(loss),
+            ctypes.byref(rid)))
+        self._pending[rid.value] = (f, out, loss[:iterations])
         return rid.value
 
     def poll(self, request: int) -> bool:
@@ -569,9 +618,9 @@ class Server:
         latency = ctypes.c_double()
         rc = _native.lib().qh_hip_server_wait(self._handle, int(request), ctypes.byref(filled),
                                               ctypes.byref(latency))
-        _, out = self._pending.pop(int(request), (None, None))
+        _, out, loss = self._pending.pop(int(request), (None, None, None))
         _native.check(rc)
-        return out, filled.value, latency.value
+        return out, (filled.value if loss is None else loss), latency.value
 
     def stats(self) -> dict:
         s = _native.ServerStats_t()

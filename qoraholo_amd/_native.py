@@ -74,6 +74,12 @@ class BpmSolveOptions_t(ctypes.Structure):
                 ("tv_weight", c_float), ("tv_inner", c_int)]
 
 
+class BpmRequestOptions_t(ctypes.Structure):
+    _fields_ = [("n", c_int), ("nz", c_int), ("count", c_int),
+                ("illuminations", POINTER(Illumination_t)), ("geometry", Geometry_t),
+                ("bpm", BpmOptions_t), ("start_iterations", c_int), ("solve", BpmSolveOptions_t)]
+
+
 class PipelineOptions_t(ctypes.Structure):
     _fields_ = [("count", c_int), ("hologram_size", c_int), ("crop", CropPlan_t),
                 ("illuminations", POINTER(Illumination_t)), ("geometry", Geometry_t),
@@ -120,6 +126,9 @@ _SIGNATURES = {
     "qh_hip_server_add_geometry": (c_int, [c_void_p, POINTER(PipelineOptions_t), c_void_p,
                                            POINTER(c_int)]),
     "qh_hip_server_submit": (c_int, [c_void_p, c_int, c_void_p, c_void_p, POINTER(ctypes.c_int64)]),
+    "qh_hip_server_add_bpm": (c_int, [c_void_p, POINTER(BpmRequestOptions_t), POINTER(c_int)]),
+    "qh_hip_server_submit_bpm": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p,
+                                         POINTER(ctypes.c_int64)]),
     "qh_hip_server_wait": (c_int, [c_void_p, ctypes.c_int64, POINTER(c_double), POINTER(c_double)]),
     "qh_hip_server_poll": (c_int, [c_void_p, ctypes.c_int64, POINTER(c_int)]),
     "qh_hip_server_stats": (c_int, [c_void_p, POINTER(ServerStats_t)]),

@@ -299,6 +299,13 @@ __global__ void __launch_bounds__(256) k_bpm_scale(float* x, long total, float s
   x[i] *= scale;
 }
 
+__global__ void __launch_bounds__(256) k_bpm_offset(const float* in, float* out, long total,
+                                                    float v) {
+  const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  out[i] = in[i] + v;
+}
+
 __global__ void __launch_bounds__(256) k_bpm_clamp(float* x, long total) {
   const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= total) return;
@@ -561,63 +568,87 @@ void bpm_scale(float* x, long total, float scale, hipStream_t s) {
   check_launch("bpm_scale");
 }
 
-void bpm_solve(BpmModel& m, const float2* fields, float* x, const BpmSolveConfig& o, double* loss,
-               hipStream_t s) {
+void bpm_offset(const float* in, float* out, long total, float v, hipStream_t s) {
+  hipLaunchKernelGGL(k_bpm_offset, dim3(blocks_for(total)), dim3(256), 0, s, in, out, total, v);
+  check_launch("bpm_offset");
+}
+
+BpmSolver::BpmSolver(BpmModel& m, const BpmSolveConfig& c)
+    : m_(m), c_(c), total_(static_cast<long>(m.nz()) * m.n() * m.n()),
+      batch_(c.batch <= 0 || c.batch >= m.count() ? m.count() : c.batch),
+      grad_(total_), z_(total_),
+      lines_(static_cast<std::size_t>(std::max(c.iterations, 1)) * m.count() * m.n()),
+      lines_host_(lines_.size()) {
+  require(c.iterations >= 0 && c.step > 0 && c.tv_weight >= 0 && c.tv_inner >= 1,
+          "bpm: invalid solver options");
+  if (batch_ == m.count()) {
+    y_.resize(total_);
+    next_.resize(total_);
+  }
+  if (c.tv_weight > 0.0f) dual_ = std::make_unique<TvState>(m.nz(), m.n(), m.n());
+  m.reserve();
+}
+
+std::size_t BpmSolver::bytes() const {
+  return grad_.bytes() + z_.bytes() + y_.bytes() + next_.bytes() + lines_.bytes() +
+         (dual_ ? dual_->bytes() : 0);
+}
+
+void BpmSolver::run(const float2* fields, float* x, double* loss, hipStream_t s) {
+  BpmModel& m = m_;
+  const BpmSolveConfig& o = c_;
   const int count = m.count();
   const int n = m.n();
-  const long total = static_cast<long>(m.nz()) * n * n;
-  const int batch = o.batch <= 0 || o.batch >= count ? count : o.batch;
+  const long total = total_;
+  const int batch = batch_;
   const float alpha = o.step / m.lipschitz();
   const bool tv = o.tv_weight > 0.0f;
   const unsigned grid = blocks_for(total);
-  DeviceBuffer<float> grad(total), z(total);
-  DeviceBuffer<double> lines(static_cast<std::size_t>(std::max(o.iterations, 1)) * count * n);
-  std::unique_ptr<TvState> dual;
-  if (tv) {
-    dual = std::make_unique<TvState>(m.nz(), n, n);
-    dual->reset(s);
-  }
+  if (tv) dual_->reset(s);
   hipLaunchKernelGGL(k_bpm_clamp, dim3(grid), dim3(256), 0, s, x, total);
   check_launch("bpm_clamp");
   // z = point - alpha grad / (last - first), then out = prox(z).
   const auto prox_step = [&](const float* point, float* out, int first, int last) {
     const float scale = 1.0f / static_cast<float>(last - first);
-    hipLaunchKernelGGL(k_bpm_descent, dim3(grid), dim3(256), 0, s, point, grad.get(), scale,
-                       alpha, total, tv ? z.get() : out, tv ? 0 : 1);
+    hipLaunchKernelGGL(k_bpm_descent, dim3(grid), dim3(256), 0, s, point, grad_.get(), scale,
+                       alpha, total, tv ? z_.get() : out, tv ? 0 : 1);
     check_launch("bpm_descent");
-    if (tv) dual->prox(z.get(), out, alpha * o.tv_weight, o.tv_inner, s);
+    if (tv) dual_->prox(z_.get(), out, alpha * o.tv_weight, o.tv_inner, s);
   };
   std::vector<int> firsts;
   if (batch == count) {
-    DeviceBuffer<float> y(total), next(total);
-    check(hipMemcpyAsync(y.get(), x, total * sizeof(float), hipMemcpyDeviceToDevice, s), "bpm copy");
+    check(hipMemcpyAsync(y_.get(), x, total * sizeof(float), hipMemcpyDeviceToDevice, s), "bpm copy");
     float tk = 1.0f;
     for (int it = 0; it < o.iterations; ++it) {
-      m.gradient(y.get(), fields, 0, count, grad.get(), lines.get() + static_cast<long>(it) * count * n, s);
-      prox_step(y.get(), next.get(), 0, count);
+      m.gradient(y_.get(), fields, 0, count, grad_.get(), lines_.get() + static_cast<long>(it) * count * n, s);
+      prox_step(y_.get(), next_.get(), 0, count);
       const float tn = fista_t(tk);
       const float mm = (tk - 1.0f) / tn;
-      hipLaunchKernelGGL(k_bpm_momentum, dim3(grid), dim3(256), 0, s, next.get(), x, y.get(), mm,
+      hipLaunchKernelGGL(k_bpm_momentum, dim3(grid), dim3(256), 0, s, next_.get(), x, y_.get(), mm,
                          total);
       check_launch("bpm_momentum");
       tk = tn;
     }
     firsts.push_back(0);
-    check(hipStreamSynchronize(s), "bpm solve");
   } else {
     for (int it = 0; it < o.iterations; ++it) {
       for (int first = 0; first < count; first += batch) {
         const int last = std::min(first + batch, count);
-        m.gradient(x, fields, first, last, grad.get(), lines.get() + static_cast<long>(it) * count * n, s);
+        m.gradient(x, fields, first, last, grad_.get(), lines_.get() + static_cast<long>(it) * count * n, s);
         prox_step(x, x, first, last);
       }
     }
     for (int first = 0; first < count; first += batch) firsts.push_back(first);
-    check(hipStreamSynchronize(s), "bpm solve");
   }
   // Data term per iteration: per batch 0.5 / count times its sum over illuminations, in order.
-  std::vector<double> h(lines.size());
-  check(hipMemcpy(h.data(), lines.get(), h.size() * sizeof(double), hipMemcpyDeviceToHost), "bpm loss");
+  const std::size_t used = static_cast<std::size_t>(o.iterations) * count * n;
+  if (used > 0) {
+    check(hipMemcpyAsync(lines_host_.get(), lines_.get(), used * sizeof(double),
+                         hipMemcpyDeviceToHost, s),
+          "bpm loss");
+  }
+  check(hipStreamSynchronize(s), "bpm solve");
+  const double* h = lines_host_.get();
   for (int it = 0; it < o.iterations; ++it) {
     double value = 0;
     for (std::size_t bi = 0; bi < firsts.size(); ++bi) {
@@ -631,8 +662,13 @@ void bpm_solve(BpmModel& m, const float2* fields, float* x, const BpmSolveConfig
       }
       value += 0.5 * sum / count;
     }
-    loss[it] = value;
+    if (loss) loss[it] = value;
   }
+}
+
+void bpm_solve(BpmModel& m, const float2* fields, float* x, const BpmSolveConfig& o, double* loss,
+               hipStream_t s) {
+  BpmSolver(m, o).run(fields, x, loss, s);
 }
 
 }  // namespace qh

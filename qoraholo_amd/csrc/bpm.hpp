@@ -5,10 +5,12 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <memory>
 #include <vector>
 
 #include "../../include/qoraholo_hip.h"
 #include "runtime.hpp"
+#include "tv.hpp"
 
 namespace qh {
 
@@ -46,6 +48,8 @@ class BpmModel {
   float phase() const { return phase_; }
   float lipschitz() const { return lipschitz_; }
   std::size_t bytes() const;
+  // Allocates the adjoint's buffers now rather than on the first gradient or vjp.
+  void reserve() { ensure_adjoint(); }
 
   // Kernel arguments, defined with the kernels.
   struct Args;
@@ -91,9 +95,36 @@ struct BpmSolveConfig {
   int tv_inner = 10;
 };
 
-// Reference src/cpu/bpm.cpp:191-243 on the device: x (contrast, device) is the starting
-// volume and receives the result; fields (count, n, n) are device field ratios; loss (host)
-// receives one data-term value per iteration.
+// out[i] = in[i] + v (in place allowed): the contrast x = ri - n_medium and back, with the
+// single float rounding of the reference's host conversion (src/api.cpp:159-163).
+void bpm_offset(const float* in, float* out, long total, float v, hipStream_t s);
+
+// Reference src/cpu/bpm.cpp:191-243 on the device, with its work buffers (gradient, prox input,
+// FISTA point, TV dual, per-row loss terms) allocated once, so that repeated solves allocate and
+// free nothing (a free would wait for all work on the device).
+class BpmSolver {
+ public:
+  BpmSolver(BpmModel& model, const BpmSolveConfig& c);
+  BpmSolver(const BpmSolver&) = delete;
+  BpmSolver& operator=(const BpmSolver&) = delete;
+  // x (contrast, device) is the starting volume and receives the result; fields (count, n, n)
+  // are device field ratios; loss (host, may be null) receives one data-term value per
+  // iteration. Returns when the stream has finished the solve.
+  void run(const float2* fields, float* x, double* loss, hipStream_t s);
+  std::size_t bytes() const;
+
+ private:
+  BpmModel& m_;
+  BpmSolveConfig c_;
+  long total_;
+  int batch_;
+  DeviceBuffer<float> grad_, z_, y_, next_;
+  DeviceBuffer<double> lines_;
+  PinnedBuffer<double> lines_host_;
+  std::unique_ptr<TvState> dual_;
+};
+
+// One solve with buffers of its own (BpmSolver constructed for the call).
 void bpm_solve(BpmModel& model, const float2* fields, float* x, const BpmSolveConfig& c,
                double* loss, hipStream_t s);
 

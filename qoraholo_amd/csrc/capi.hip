@@ -622,6 +622,52 @@ int qh_hip_server_submit(qh_server* s, int geometry, const float* holograms, flo
   });
 }
 
+int qh_hip_server_add_bpm(qh_server* s, const qh_bpm_request_options* o, int* geometry) {
+  return guarded([&] {
+    require(s && o && geometry, "Server: null argument");
+    const qh_bpm_solve_options& v = o->solve;
+    require(v.iterations >= 0 && v.step > 0 && v.tv_weight >= 0 && v.tv_inner >= 1 && v.batch >= 0,
+            "Server: invalid BPM solver options");
+    require(v.tv_weight == 0 || o->n <= 512, "Server: the TV prox handles n <= 512");
+    require(o->start_iterations >= 0, "Server: start iterations must not be negative");
+    BpmInversionSpec spec;
+    spec.tables = checked_bpm_tables(o->nz, o->n, o->count, o->illuminations, &o->geometry,
+                                     &o->bpm, "Server");
+    spec.chunk = o->bpm.chunk;
+    spec.checkpoint = o->bpm.checkpoint;
+    qh_odt_options start{};
+    start.nz = o->nz;
+    start.iterations = o->start_iterations;
+    start.solver = 0;
+    start.gp_transform = 0;
+    start.tv_weight = 1e-2f;
+    start.tv_inner = 10;
+    spec.grid = checked_grid(o->n, o->count, o->illuminations, o->geometry, start, "Server");
+    spec.start = config_from(start, o->geometry);
+    spec.illuminations.resize(2 * static_cast<std::size_t>(o->count));
+    for (int a = 0; a < o->count; ++a) {
+      spec.illuminations[2 * a] = o->illuminations[a].fy;
+      spec.illuminations[2 * a + 1] = o->illuminations[a].fx;
+    }
+    spec.solve.iterations = v.iterations;
+    spec.solve.batch = v.batch;
+    spec.solve.step = v.step;
+    spec.solve.tv_weight = v.tv_weight;
+    spec.solve.tv_inner = v.tv_inner;
+    spec.n_medium = o->geometry.n_medium;
+    require_device();
+    *geometry = s->impl->add_bpm(spec);
+  });
+}
+
+int qh_hip_server_submit_bpm(qh_server* s, int geometry, const qh_c64* fields, float* ri_out,
+                             double* loss, int64_t* id) {
+  return guarded([&] {
+    require(s && fields && ri_out && id, "Server: null argument");
+    *id = s->impl->submit(geometry, fields, ri_out, loss);
+  });
+}
+
 int qh_hip_server_wait(qh_server* s, int64_t id, double* filled, double* latency_ms) {
   int status = QH_OK;
   const int rc = guarded([&] {

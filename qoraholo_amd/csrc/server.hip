@@ -19,6 +19,42 @@ std::size_t free_device_bytes() {
   return free_b;
 }
 
+// A reconstruction slot: a Pipeline of the geometry with its background fields set.
+class ReconWorker final : public SlotWorker {
+ public:
+  struct Spec {
+    qh_pipeline_options options;
+    std::vector<qh_illumination> illuminations;
+    std::vector<float> backgrounds;
+  };
+  explicit ReconWorker(const Spec& spec) : pipeline_(spec.options) {
+    // Slots launch kernel by kernel: a graph capture on one slot's thread would be invalidated
+    // by another slot being set up (allocations, synchronous copies) meanwhile, and with
+    // several slots in flight the launch gaps are covered by the other slots' work.
+    pipeline_.use_graph(false);
+    pipeline_.set_background(spec.backgrounds.data());
+  }
+  void run(const void* input, float* out, double*) override {
+    pipeline_.run(static_cast<const float*>(input), out);
+  }
+  double filled() const override { return pipeline_.filled(); }
+
+ private:
+  Pipeline pipeline_;
+};
+
+// A BPM inversion slot.
+class BpmWorker final : public SlotWorker {
+ public:
+  explicit BpmWorker(const BpmInversionSpec& spec) : inversion_(spec) {}
+  void run(const void* input, float* out, double* extra) override {
+    inversion_.run(static_cast<const float2*>(input), out, extra);
+  }
+
+ private:
+  BpmInversion inversion_;
+};
+
 }  // namespace
 
 Server::Server(std::size_t budget_bytes, int max_slots)
@@ -53,27 +89,32 @@ void Server::reset_stats() {
 int Server::add_geometry(const qh_pipeline_options& o, const float* backgrounds) {
   check_pipeline_options(o);
   require(backgrounds != nullptr, "Server: background holograms are required");
-  auto g = std::make_unique<Geometry>();
-  g->options = o;
-  g->illuminations.assign(o.illuminations, o.illuminations + o.count);
-  g->options.illuminations = g->illuminations.data();
+  auto spec = std::make_shared<ReconWorker::Spec>();
+  spec->options = o;
+  spec->illuminations.assign(o.illuminations, o.illuminations + o.count);
+  spec->options.illuminations = spec->illuminations.data();
   const std::size_t image = static_cast<std::size_t>(o.hologram_size) * o.hologram_size;
-  g->backgrounds.assign(backgrounds, backgrounds + image * o.count);
+  spec->backgrounds.assign(backgrounds, backgrounds + image * o.count);
+  return add_kind([spec] { return std::make_unique<ReconWorker>(*spec); });
+}
+
+int Server::add_bpm(const BpmInversionSpec& spec) {
+  auto shared = std::make_shared<BpmInversionSpec>(spec);
+  return add_kind([shared] { return std::make_unique<BpmWorker>(*shared); });
+}
+
+int Server::add_kind(std::function<std::unique_ptr<SlotWorker>()> make) {
+  auto g = std::make_unique<Geometry>();
+  g->make = std::move(make);
   // The first slot is built now: it measures the footprint and fails early when it does not fit.
+  check(hipDeviceSynchronize(), "Server::add_geometry");
   const std::size_t before = free_device_bytes();
   auto slot = std::make_unique<Slot>();
-  slot->pipeline = std::make_unique<Pipeline>(g->options);
-  // Slots launch kernel by kernel: a graph capture on one slot's thread would be invalidated
-  // by another slot being set up (allocations, synchronous copies) meanwhile, and with several
-  // slots in flight the launch gaps are covered by the other slots' work.
-  slot->pipeline->use_graph(false);
-  slot->pipeline->set_background(g->backgrounds.data());
+  slot->pipeline = g->make();
   check(hipDeviceSynchronize(), "Server::add_geometry");
   const std::size_t after = free_device_bytes();
   slot->bytes = before > after ? before - after : 0;
   g->slot_bytes = std::max<std::size_t>(slot->bytes, 1);
-  g->nz = slot->pipeline->nz();
-  g->n = slot->pipeline->n();
   std::lock_guard<std::mutex> lock(mu_);
   if (used_bytes_ + slot->bytes > budget_) {
     throw OutOfMemory("Server: one slot of this geometry needs " + std::to_string(slot->bytes) +
@@ -90,11 +131,12 @@ int Server::add_geometry(const qh_pipeline_options& o, const float* backgrounds)
   return id;
 }
 
-std::int64_t Server::submit(int geometry, const float* holograms, float* ri_out) {
-  require(holograms != nullptr && ri_out != nullptr, "Server::submit: null buffer");
+std::int64_t Server::submit(int geometry, const void* input, float* ri_out, double* extra) {
+  require(input != nullptr && ri_out != nullptr, "Server::submit: null buffer");
   auto r = std::make_shared<Request>();
-  r->holograms = holograms;
+  r->input = input;
   r->out = ri_out;
+  r->extra = extra;
   r->arrival = std::chrono::steady_clock::now();
   {
     std::lock_guard<std::mutex> lock(mu_);
@@ -147,9 +189,7 @@ bool Server::try_admit(const std::shared_ptr<Request>& r) {
   }
   auto slot = std::make_unique<Slot>();
   slot->geometry = r->geometry;
-  slot->pipeline = std::make_unique<Pipeline>(g.options);
-  slot->pipeline->use_graph(false);  // as in add_geometry
-  slot->pipeline->set_background(g.backgrounds.data());
+  slot->pipeline = g.make();
   slot->bytes = g.slot_bytes;
   used_bytes_ += slot->bytes;
   ++g.slots;
@@ -201,7 +241,7 @@ void Server::slot_loop(Slot* slot) {
     std::string error;
     double filled = 0;
     try {
-      slot->pipeline->run(r->holograms, r->out);
+      slot->pipeline->run(r->input, r->out, r->extra);
       filled = slot->pipeline->filled();
     } catch (const InvalidArgument& e) {
       status = QH_ERR_INVALID_ARGUMENT;

@@ -1,14 +1,17 @@
-// Served reconstruction loop: requests of different geometries (hologram count and size,
-// volume size, solver) arrive on host threads; each geometry keeps its background fields, plans
-// and solver state resident in device-resident slots; a dispatcher admits queued requests into
-// free slots, creating slots while the HBM budget allows, so requests that arrive while others
-// are on the device start as soon as a slot frees up and run concurrently on their own streams.
+// Served loop: requests of different kinds and geometries (reconstructions of holograms of a
+// given count and size into a volume of a given size and solver; BPM inversions of field ratios
+// of a given volume and illumination set) arrive on host threads; each geometry keeps its
+// background fields, plans and solver state resident in device-resident slots; a dispatcher
+// admits queued requests into free slots, creating slots while the HBM budget allows, so
+// requests that arrive while others are on the device start as soon as a slot frees up and run
+// concurrently on their own streams.
 #pragma once
 
 #include <chrono>
 #include <condition_variable>
 #include <cstdint>
 #include <deque>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -17,9 +20,22 @@
 #include <vector>
 
 #include "../../include/qoraholo_hip.h"
+#include "bpm_inversion.hpp"
 #include "pipeline.hpp"
 
 namespace qh {
+
+// What a slot runs: one request at a time, host input to host output, on device state of its
+// own that it allocated when it was built.
+class SlotWorker {
+ public:
+  virtual ~SlotWorker() = default;
+  // input: the request's host input; out: its host result; extra: an optional host output of
+  // the request kind (the loss of each BPM iteration).
+  virtual void run(const void* input, float* out, double* extra) = 0;
+  // Filled fraction of the last reconstruction's k-space (0 for other kinds).
+  virtual double filled() const { return 0.0; }
+};
 
 class Server {
  public:
@@ -28,11 +44,16 @@ class Server {
   Server(const Server&) = delete;
   Server& operator=(const Server&) = delete;
 
-  // Registers a geometry with its background holograms (count, N, N); returns its id.
+  // Registers a reconstruction geometry with its background holograms (count, N, N); returns
+  // its id.
   int add_geometry(const qh_pipeline_options& o, const float* backgrounds);
-  // Queues a request; holograms (count, N, N) and ri_out (nz, n, n) are host memory that must
-  // stay valid until wait() returns for this id.
-  std::int64_t submit(int geometry, const float* holograms, float* ri_out);
+  // Registers a BPM inversion geometry; returns its id.
+  int add_bpm(const BpmInversionSpec& spec);
+  // Queues a request; its input (holograms (count, N, N) of a reconstruction geometry, field
+  // ratios (count, n, n) complex of a BPM one), ri_out (nz, n, n) and extra (the per-iteration
+  // loss of a BPM request, may be null) are host memory that must stay valid until wait()
+  // returns for this id.
+  std::int64_t submit(int geometry, const void* input, float* ri_out, double* extra = nullptr);
   // Blocks until the request is done; returns its status and message.
   int wait(std::int64_t id, double* filled, double* latency_ms, std::string* error);
   bool done(std::int64_t id);
@@ -41,18 +62,16 @@ class Server {
 
  private:
   struct Geometry {
-    qh_pipeline_options options;
-    std::vector<qh_illumination> illuminations;
-    std::vector<float> backgrounds;
+    std::function<std::unique_ptr<SlotWorker>()> make;
     std::size_t slot_bytes = 0;  // device bytes of one slot, measured on the first
     int slots = 0;
-    int nz = 0, n = 0;
   };
   struct Request {
     std::int64_t id;
     int geometry;
-    const float* holograms;
+    const void* input;
     float* out;
+    double* extra;
     std::chrono::steady_clock::time_point arrival;
     bool finished = false;
     int status = QH_OK;
@@ -62,7 +81,7 @@ class Server {
   };
   struct Slot {
     int geometry;
-    std::unique_ptr<Pipeline> pipeline;
+    std::unique_ptr<SlotWorker> pipeline;
     std::size_t bytes = 0;
     std::shared_ptr<Request> work;
     bool busy = false;
@@ -71,6 +90,7 @@ class Server {
     std::thread thread;
   };
 
+  int add_kind(std::function<std::unique_ptr<SlotWorker>()> make);
   void dispatch_loop();
   void slot_loop(Slot* slot);
   bool try_admit(const std::shared_ptr<Request>& r);

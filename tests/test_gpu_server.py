@@ -83,3 +83,74 @@ def test_bad_requests_are_rejected_alone(device, qh, mixed):
         server.submit(gids[0], d.holo[:, :64])
     rid = server.submit(gids[0], d.holo)
     assert np.array_equal(server.wait(rid)[0], alone)
+
+
+# ---- BPM inversion requests (the served BPM configuration at test sizes) ----
+
+BPM_CASES = ((48, 9, 3, 0.0), (64, 13, 4, 1e-2))  # n, illuminations, iterations, TV weight
+
+
+@pytest.fixture(scope="module")
+def bpm_mixed(qh):
+    """Two BPM geometries and, for each, the result of the per-call chain the served request
+    replaces (examples/bpm_figure.py:74-88): rytov, a 2-iteration GP start, bpm_reconstruct."""
+    from qoraholo_amd import workloads
+
+    cases = []
+    for n, angles, iterations, tv in BPM_CASES:
+        req = workloads.bpm_request(n, angles, seed=n)
+        g, il, f = req.geometry, req.illuminations, req.fields
+        start = qh.reconstruct(qh.rytov(f), il, g, iterations=2)
+        ri, loss = qh.bpm_reconstruct(f, il, g, start, iterations=iterations, tv_weight=tv,
+                                      tv_inner=4, return_loss=True)
+        kw = dict(start_iterations=2, iterations=iterations, tv_weight=tv, tv_inner=4)
+        cases.append((req, kw, ri, loss))
+    return cases
+
+
+def test_bpm_requests_equal_the_per_call_chain_alone_and_in_the_mix(device, qh, mixed, bpm_mixed):
+    """Each served BPM inversion gives the per-call chain's index volume and loss bit for bit,
+    alone and interleaved with another BPM geometry and reconstruction requests."""
+    server = qh.Server(max_slots=4)
+    bids = [server.add_bpm(req.fields.shape[1], req.illuminations, req.geometry, **kw)
+            for req, kw, _, _ in bpm_mixed]
+    print("served bpm alone", flush=True)
+    for k, gid in enumerate(bids):
+        ri, loss, latency = server.wait(server.submit_bpm(gid, bpm_mixed[k][0].fields))
+        assert np.array_equal(ri, bpm_mixed[k][2]), k
+        assert np.array_equal(loss, bpm_mixed[k][3]), k
+        assert latency > 0
+    print("served bpm mixed", flush=True)
+    rids = add_all(server, mixed[:2])
+    order = [("b", 1), ("r", 0), ("b", 0), ("b", 1), ("r", 1), ("b", 0)]
+    ids = []
+    for kind, k in order:
+        if kind == "b":
+            ids.append((kind, k, server.submit_bpm(bids[k], bpm_mixed[k][0].fields)))
+        else:
+            ids.append((kind, k, server.submit(rids[k], mixed[k][0].holo)))
+    for kind, k, rid in ids:
+        out, extra, _ = server.wait(rid)
+        if kind == "b":
+            assert np.array_equal(out, bpm_mixed[k][2]), (kind, k)
+            assert np.array_equal(extra, bpm_mixed[k][3]), (kind, k)
+        else:
+            assert np.array_equal(out, mixed[k][1]), (kind, k)
+    s = server.stats()
+    assert s["completed"] == len(order) + len(bids) and s["failed"] == 0
+    assert s["slots"] <= 4
+
+
+def test_bpm_requests_are_checked(device, qh, bpm_mixed):
+    server = qh.Server(max_slots=2)
+    req, kw, ri, _ = bpm_mixed[0]
+    gid = server.add_bpm(req.fields.shape[1], req.illuminations, req.geometry, **kw)
+    with pytest.raises(ValueError):
+        server.submit_bpm(gid, req.fields[:, :16])
+    with pytest.raises(ValueError):
+        server.submit(gid, req.fields.real)
+    with pytest.raises(ValueError):
+        server.add_bpm(req.fields.shape[1], req.illuminations, req.geometry, iterations=2,
+                       step=-1.0)
+    rid = server.submit_bpm(gid, req.fields)
+    assert np.array_equal(server.wait(rid)[0], ri)
