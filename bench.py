@@ -388,8 +388,8 @@ def serving_comparison(qh, w, plain_requests=3, serial_requests=5, served_reques
     - plain: the per-call chain a user of the reference's API runs (extract_fields for sample
       and background, field_ratio on the host, rytov, reconstruct), serially;
     - pipeline: Pipeline.run, serially (background fields resident);
-    - served: the Server, a burst of requests submitted at once (throughput), then a Poisson
-      stream at 80% of that throughput (latency under load)."""
+    - served: the Server with two slots, a burst of requests submitted at once (throughput),
+      then a Poisson stream at 80% of that throughput (latency under load)."""
     g = w.geometry
     kw = dict(iterations=ITERATIONS, solver="tv", tv_weight=TV_WEIGHT, tv_inner=INNER)
     crop = dict(center=w.center, radius=w.radius, output_size=N)
@@ -427,10 +427,12 @@ def serving_comparison(qh, w, plain_requests=3, serial_requests=5, served_reques
     log(f"serial pipeline: {out['pipeline']}")
     del p
 
-    server = qh.Server(max_slots=4)
+    # Two slots: one computes while the other moves its holograms in and its volume out; more
+    # slots of the same geometry only share the device between more FGP passes.
+    server = qh.Server(max_slots=2)
     gid = server.add_geometry(ANGLES, 2 * N, w.center, w.illuminations, g, w.backgrounds,
                               radius=w.radius, output_size=N, **kw)
-    for rid in [server.submit(gid, w.holograms) for _ in range(4)]:
+    for rid in [server.submit(gid, w.holograms) for _ in range(2)]:
         server.wait(rid)
     server.reset_stats()
     t0 = time.perf_counter()

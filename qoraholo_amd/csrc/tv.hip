@@ -47,13 +47,14 @@ __device__ __forceinline__ int xcd_tile() {
 // Row access. VEC: nx is a multiple of VX and VX of 2, so a lane's VX values are all inside the
 // row or all outside it and every access is whole float4s (float2s when VX is not a multiple of
 // 4) under one predicate per lane; otherwise element by element.
-template <int VX, bool VEC>
+// XNEG: xb may be negative (the left halo of an x tile); other callers start rows at column 0.
+template <int VX, bool VEC, bool XNEG = false>
 __device__ __forceinline__ void load_row(const float* base, const TvArgs& a, int z, int y, int xb,
                                          float (&out)[VX]) {
   const bool inside = z >= 0 && z < a.nz && y >= 0 && y < a.ny;
   if constexpr (VEC) {
     static_assert(VX % 2 == 0, "whole float2s");
-    if (inside && xb >= 0 && xb < a.nx) {
+    if (inside && (!XNEG || xb >= 0) && xb < a.nx) {
       const float* p = base + (static_cast<long>(z) * a.ny + y) * a.nx + xb;
       if constexpr (VX % 4 == 0) {
 #pragma unroll
@@ -83,7 +84,7 @@ __device__ __forceinline__ void load_row(const float* base, const TvArgs& a, int
     const long row = (static_cast<long>(z) * a.ny + y) * a.nx;
 #pragma unroll
     for (int v = 0; v < VX; ++v) {
-      if (xb + v >= 0 && xb + v < a.nx) out[v] = base[row + xb + v];
+      if ((!XNEG || xb + v >= 0) && xb + v < a.nx) out[v] = base[row + xb + v];
     }
   }
 }
@@ -302,7 +303,7 @@ struct PairArgs {
 };
 
 // z = max(b - lambda L(r), 0) on one row of plane zp (the expression order of primal_plane).
-template <int VX>
+template <int VX, bool XNEG>
 __device__ __forceinline__ void pair_primal(const PairArgs& a, int zp, int y, int xb,
                                             const float (&R)[3][VX], const float (&rz_below)[VX],
                                             const float (&ry_up)[VX], const float (&B)[VX],
@@ -318,7 +319,7 @@ __device__ __forceinline__ void pair_primal(const PairArgs& a, int zp, int y, in
     if (y > 0) s -= ry_up[v];
     if (x < a.nx - 1) s += R[2][v];
     if (x > 0) s -= (v == 0 ? left : R[2][v - 1]);
-    Z[v] = (y >= 0 && y < a.ny && x >= 0 && x < a.nx && zp >= 0 && zp < a.nz)
+    Z[v] = (y >= 0 && y < a.ny && (!XNEG || x >= 0) && x < a.nx && zp >= 0 && zp < a.nz)
                ? tv_primal(B[v], a.lambda, s)
                : 0.0f;
   }
@@ -407,12 +408,13 @@ __device__ __forceinline__ void lds_get(const float* buf, int row, int nx, int x
   }
 }
 
-// VEC: see load_row.
-template <int VX, int NW, bool VEC>
+// VEC: see load_row. XT: rows are cut into x tiles (more than one per row).
+template <int VX, int NW, bool VEC, bool XT>
 __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
   constexpr int TOP = 2;  // halo rows above the output rows (and two below)
   constexpr int T = NW - 4;
-  constexpr int LW = 64 * VX;  // columns one wave holds: the LDS row width
+  // LDS row width: the row when a wave holds it whole, else the columns one wave holds.
+  const int LW = XT ? 64 * VX : a.nx;
   extern __shared__ float4 lds4[];
   float* lds = reinterpret_cast<float*>(lds4);
   float* ry = lds;              // y component of r_k of each row
@@ -430,18 +432,18 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
   // output columns; each tile's wave also holds one lane of columns on either side, which two
   // steps of the 7-point stencil need, and stores only its own columns.
   const int tile = xcd_tile();
-  const int tx = tile % a.tiles_x;
-  const int rest = tile / a.tiles_x;
-  const int tiles_y = static_cast<int>(gridDim.x) / (chunks * a.tiles_x);
+  const int tx = XT ? tile % a.tiles_x : 0;
+  const int rest = XT ? tile / a.tiles_x : tile;
+  const int tiles_y = static_cast<int>(gridDim.x) / (chunks * (XT ? a.tiles_x : 1));
   const int y0 = (rest % tiles_y) * T;
   const int z0 = (rest / tiles_y) * a.zlen;
   const int z1 = min(z0 + a.zlen, a.nz);
   const int y = y0 - TOP + w;
-  const int xo = tx * a.xt;                         // first output column of the tile
-  const int x0 = a.tiles_x > 1 ? xo - VX : 0;       // column of lane 0
-  const int xl = lane * VX;                         // column within the LDS row
+  const int xo = tx * a.xt;            // first output column of the tile
+  const int x0 = XT ? xo - VX : 0;     // column of lane 0
+  const int xl = lane * VX;            // column within the LDS row
   const int xb = x0 + xl;
-  const bool out_col = xb >= xo && xb < min(xo + a.xt, a.nx);
+  const bool out_col = !XT || (xb >= xo && xb < min(xo + a.xt, a.nx));
   const int wu = w > 0 ? w - 1 : 0;       // the first and last waves read their own row: their
   const int wd = w < NW - 1 ? w + 1 : w;  // results only feed other halo values, never a store
   const bool out_row = w >= TOP && w < TOP + T && y < a.ny;
@@ -450,15 +452,15 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
   auto load_raw = [&](int f, float (&Pk)[3][VX], float (&Pm)[3][VX], float (&B)[VX]) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      load_row<VX, VEC>(a.p_k + c * a.vol, bd, f, y, xb, Pk[c]);
+      load_row<VX, VEC, XT>(a.p_k + c * a.vol, bd, f, y, xb, Pk[c]);
       if (!a.first) {
-        load_row<VX, VEC>(a.p_km1 + c * a.vol, bd, f, y, xb, Pm[c]);
+        load_row<VX, VEC, XT>(a.p_km1 + c * a.vol, bd, f, y, xb, Pm[c]);
       } else {
 #pragma unroll
         for (int v = 0; v < VX; ++v) Pm[c][v] = 0.0f;
       }
     }
-    load_row<VX, VEC>(a.b, bd, f, y, xb, B);
+    load_row<VX, VEC, XT>(a.b, bd, f, y, xb, B);
   };
   // r_k = p_k + m1 (p_k - p_{k-1}), or p_k on the first step.
   auto extrapolate_k = [&](const float (&Pk)[3][VX], const float (&Pm)[3][VX], float (&R)[3][VX]) {
@@ -473,9 +475,9 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       float pm[VX];
-      load_row<VX, VEC>(a.p_k + c * a.vol, bd, f, y, xb, Pk[c]);
+      load_row<VX, VEC, XT>(a.p_k + c * a.vol, bd, f, y, xb, Pk[c]);
       if (!a.first) {
-        load_row<VX, VEC>(a.p_km1 + c * a.vol, bd, f, y, xb, pm);
+        load_row<VX, VEC, XT>(a.p_km1 + c * a.vol, bd, f, y, xb, pm);
       } else {
 #pragma unroll
         for (int v = 0; v < VX; ++v) pm[v] = 0.0f;
@@ -483,7 +485,7 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
 #pragma unroll
       for (int v = 0; v < VX; ++v) R[c][v] = a.first ? Pk[c][v] : Pk[c][v] + a.m1 * (Pk[c][v] - pm[v]);
     }
-    load_row<VX, VEC>(a.b, bd, f, y, xb, B);
+    load_row<VX, VEC, XT>(a.b, bd, f, y, xb, B);
   };
 
   // Prologue: r_k on plane z0 - pre (z component) and z_k on plane z0 - pre + 1.
@@ -491,9 +493,9 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
   float rz_m2[VX];
   {
     float pk[VX], pm[VX];
-    load_row<VX, VEC>(a.p_k, bd, zs - 1, y, xb, pk);
+    load_row<VX, VEC, XT>(a.p_k, bd, zs - 1, y, xb, pk);
     if (!a.first) {
-      load_row<VX, VEC>(a.p_km1, bd, zs - 1, y, xb, pm);
+      load_row<VX, VEC, XT>(a.p_km1, bd, zs - 1, y, xb, pm);
     } else {
 #pragma unroll
       for (int v = 0; v < VX; ++v) pm[v] = 0.0f;
@@ -508,7 +510,7 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
   {
     float up[VX];
     lds_get<VX, VEC>(ry, wu, LW, xl, up);
-    pair_primal<VX>(a, zs, y, xb, rk1, rz_m2, up, b1, zk1);
+    pair_primal<VX, XT>(a, zs, y, xb, rk1, rz_m2, up, b1, zk1);
   }
   lds_put<VX, VEC>(zk, w, LW, xl, zk1);
   float rq0[3][VX], zq0[VX], zdn_k[VX], zdn_q[VX];
@@ -546,7 +548,7 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
     if (need_zk) {
       float up[VX];
       lds_get<VX, VEC>(ry, wu, LW, xl, up);
-      pair_primal<VX>(a, f, y, xb, rk2, rk1[0], up, b2, zk2);
+      pair_primal<VX, XT>(a, f, y, xb, rk2, rk1[0], up, b2, zk2);
     }
     // p_{k+1} and r_{k+1} on plane t + 1.
     float rq1[3][VX] = {};
@@ -571,7 +573,7 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
     if (need_zq) {
       float up[VX];
       lds_get<VX, VEC>(rqy, wu, LW, xl, up);
-      pair_primal<VX>(a, t + 1, y, xb, rq1, rq0[0], up, b1, zq1);
+      pair_primal<VX, XT>(a, t + 1, y, xb, rq1, rq0[0], up, b1, zq1);
     }
     lds_put<VX, VEC>(zq, w, LW, xl, zq1);
     __syncthreads();  // (3) z_{k+1} of plane t + 1 visible
@@ -753,22 +755,25 @@ namespace {
 
 // Rows per block of the two-step kernel: NW waves, NW - 4 output rows, as many waves as the
 // registers of VX floats per lane leave room for in one block per CU. Rows up to 256 wide are
-// held whole by a wave; wider rows are cut into x tiles of at most kPairTileX output columns
-// (one lane of 4 columns of halo on either side), so every width runs the 4-column variant.
+// held whole by a wave; wider rows are cut into x tiles of at most 62 kTileVX output columns
+// (one lane of kTileVX columns of halo on either side), balanced. Tiles of 2 columns per lane
+// run 16 waves (12 output rows) and leave fewer lanes idle than tiles of 4 (384: 4 tiles of 96
+// columns, 52 of 64 lanes busy, against 2 tiles of 192, 50 lanes, and 8 output rows).
 int pair_waves(int vx) { return vx == 1 ? 16 : (vx == 2 ? 16 : 12); }
-constexpr int kPairTileX = 64 * 4 - 2 * 4;
+constexpr int kTileVX = 2;
+constexpr int kPairTileX = 62 * kTileVX;
 
-template <int VX, int NW>
+template <int VX, int NW, bool XT>
 void launch_pair(const PairArgs& a, int chunks, dim3 grid, hipStream_t s) {
-  // Four row buffers of NW x 64 VX floats: at most 48 KiB.
-  const std::size_t lds = 4 * static_cast<std::size_t>(NW) * 64 * VX * sizeof(float);
+  // Four row buffers of NW rows of nx floats (64 VX per x tile): at most 48 KiB.
+  const std::size_t lds = 4 * static_cast<std::size_t>(NW) * (XT ? 64 * VX : a.nx) * sizeof(float);
   if constexpr (VX % 2 == 0) {
     if (rows_vec<VX>(a.nx)) {
-      hipLaunchKernelGGL((k_fgp_pair<VX, NW, true>), grid, dim3(NW * 64), lds, s, a, chunks);
+      hipLaunchKernelGGL((k_fgp_pair<VX, NW, true, XT>), grid, dim3(NW * 64), lds, s, a, chunks);
       return;
     }
   }
-  hipLaunchKernelGGL((k_fgp_pair<VX, NW, false>), grid, dim3(NW * 64), lds, s, a, chunks);
+  hipLaunchKernelGGL((k_fgp_pair<VX, NW, false, XT>), grid, dim3(NW * 64), lds, s, a, chunks);
 }
 
 }  // namespace
@@ -779,12 +784,13 @@ void fgp_pair(const float* b, const float* p_k, const float* p_km1, float* p_k1,
               int nz, int ny, int nx, float lambda, float step, float m1, bool first, float m2,
               hipStream_t s) {
   require(fgp_pair_supported(nx), "fgp_pair: rows up to 512 wide");
-  const int vx = std::min(lanes_x(nx), 4);
+  int vx = std::min(lanes_x(nx), 4);
   int tiles_x = 1;
   int xt = nx;
   if (nx > 64 * 4) {
+    vx = kTileVX;
     tiles_x = (nx + kPairTileX - 1) / kPairTileX;
-    xt = ((nx + tiles_x - 1) / tiles_x + 3) / 4 * 4;  // balanced tiles, whole lanes
+    xt = ((nx + tiles_x - 1) / tiles_x + vx - 1) / vx * vx;  // balanced tiles, whole lanes
   }
   const int nw = pair_waves(vx);
   const int rows = nw - 4;
@@ -797,10 +803,14 @@ void fgp_pair(const float* b, const float* p_k, const float* p_km1, float* p_k1,
              nz, ny,     nx,    lambda, step,        m1, first ? 1 : 0, m2, zlen, xt, tiles_x};
   const int chunks = (nz + zlen - 1) / zlen;
   const dim3 grid(static_cast<unsigned>(tiles_x * tiles_y * chunks));
-  switch (vx) {
-    case 1: launch_pair<1, 16>(a, chunks, grid, s); break;
-    case 2: launch_pair<2, 16>(a, chunks, grid, s); break;
-    default: launch_pair<4, 12>(a, chunks, grid, s); break;
+  if (tiles_x > 1) {
+    launch_pair<kTileVX, kTileVX == 2 ? 16 : 12, true>(a, chunks, grid, s);
+  } else {
+    switch (vx) {
+      case 1: launch_pair<1, 16, false>(a, chunks, grid, s); break;
+      case 2: launch_pair<2, 16, false>(a, chunks, grid, s); break;
+      default: launch_pair<4, 12, false>(a, chunks, grid, s); break;
+    }
   }
   check_launch("fgp_pair");
 }
