@@ -331,13 +331,13 @@ void launch_with(Kind k, Args a, unsigned grid, std::size_t lds, hipStream_t s) 
   }
 }
 
-// Rows: one-wave blocks (8 lines of 24 threads, three waves, for 384). Columns: at least 16
-// lines per block, so that each block row is a whole 128-byte segment (blocks of 8 lines would
-// share every cache line with a block that may run on another XCD).
+// Rows: blocks of 8 lines (2 waves at 256; 4 or 16 lines were no faster). Columns: at least
+// 16 lines per block, so that each block row is a whole 128-byte segment (blocks of 8 lines
+// would share every cache line with a block that may run on another XCD).
 template <bool ROWS, int N1, int N2>
 void launch_reg(Kind k, Args a, hipStream_t s) {
   constexpr int kT = N1 > N2 ? N1 : N2;
-  constexpr int kRows = 64 % kT == 0 ? 64 : 8 * kT;
+  constexpr int kRows = 8 * kT;
   constexpr int kCols = 16 * kT > 256 ? 16 * kT : 256;
   using E = RegLines<N1, N2, !ROWS, ROWS ? kRows : kCols>;
   a.g.lines_per_block = E::LINES;
