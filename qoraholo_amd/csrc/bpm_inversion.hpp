@@ -40,6 +40,7 @@ class BpmInversion {
   int n() const { return model_.n(); }
   int nz() const { return model_.nz(); }
   int count() const { return model_.count(); }
+  int chunk() const { return model_.chunk(); }
 
  private:
   Stream stream_;

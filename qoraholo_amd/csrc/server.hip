@@ -50,6 +50,7 @@ class BpmWorker final : public SlotWorker {
   void run(const void* input, float* out, double* extra) override {
     inversion_.run(static_cast<const float2*>(input), out, extra);
   }
+  int chunk() const { return inversion_.chunk(); }
 
  private:
   BpmInversion inversion_;
@@ -100,7 +101,14 @@ int Server::add_geometry(const qh_pipeline_options& o, const float* backgrounds)
 
 int Server::add_bpm(const BpmInversionSpec& spec) {
   auto shared = std::make_shared<BpmInversionSpec>(spec);
-  return add_kind([shared] { return std::make_unique<BpmWorker>(*shared); });
+  // An automatic illumination chunk is chosen from the free memory when the first slot is
+  // built (inside add_kind, before the geometry is visible to the dispatcher) and kept for every
+  // later slot, so that all slots have the footprint admission measured on the first.
+  return add_kind([shared] {
+    auto w = std::make_unique<BpmWorker>(*shared);
+    if (shared->chunk == 0) shared->chunk = w->chunk();
+    return w;
+  });
 }
 
 int Server::add_kind(std::function<std::unique_ptr<SlotWorker>()> make) {

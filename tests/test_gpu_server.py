@@ -154,3 +154,10 @@ def test_bpm_requests_are_checked(device, qh, bpm_mixed):
                        step=-1.0)
     rid = server.submit_bpm(gid, req.fields)
     assert np.array_equal(server.wait(rid)[0], ri)
+
+
+def test_bpm_geometry_over_budget_is_refused(device, qh, bpm_mixed):
+    server = qh.Server(budget_bytes=1 << 20)
+    req, kw, _, _ = bpm_mixed[0]
+    with pytest.raises(qh.OutOfDeviceMemory):
+        server.add_bpm(req.fields.shape[1], req.illuminations, req.geometry, **kw)
