@@ -115,6 +115,9 @@ int Server::add_kind(std::function<std::unique_ptr<SlotWorker>()> make) {
   auto g = std::make_unique<Geometry>();
   g->make = std::move(make);
   // The first slot is built now: it measures the footprint and fails early when it does not fit.
+  // The lock keeps the dispatcher from building slots meanwhile (it builds them under the same
+  // lock), so the change in free memory is this slot's alone; running slots allocate nothing.
+  std::lock_guard<std::mutex> lock(mu_);
   check(hipDeviceSynchronize(), "Server::add_geometry");
   const std::size_t before = free_device_bytes();
   auto slot = std::make_unique<Slot>();
@@ -123,7 +126,6 @@ int Server::add_kind(std::function<std::unique_ptr<SlotWorker>()> make) {
   const std::size_t after = free_device_bytes();
   slot->bytes = before > after ? before - after : 0;
   g->slot_bytes = std::max<std::size_t>(slot->bytes, 1);
-  std::lock_guard<std::mutex> lock(mu_);
   if (used_bytes_ + slot->bytes > budget_) {
     throw OutOfMemory("Server: one slot of this geometry needs " + std::to_string(slot->bytes) +
                       " bytes, over the HBM budget");
