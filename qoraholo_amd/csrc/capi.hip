@@ -618,7 +618,7 @@ int qh_hip_server_submit(qh_server* s, int geometry, const float* holograms, flo
                          int64_t* id) {
   return guarded([&] {
     require(s && holograms && ri_out && id, "Server: null argument");
-    *id = s->impl->submit(geometry, holograms, ri_out);
+    *id = s->impl->submit(geometry, Server::kReconstruction, holograms, ri_out);
   });
 }
 
@@ -664,7 +664,7 @@ int qh_hip_server_submit_bpm(qh_server* s, int geometry, const qh_c64* fields, f
                              double* loss, int64_t* id) {
   return guarded([&] {
     require(s && fields && ri_out && id, "Server: null argument");
-    *id = s->impl->submit(geometry, fields, ri_out, loss);
+    *id = s->impl->submit(geometry, Server::kBpm, fields, ri_out, loss);
   });
 }
 

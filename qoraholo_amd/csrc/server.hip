@@ -96,7 +96,7 @@ int Server::add_geometry(const qh_pipeline_options& o, const float* backgrounds)
   spec->options.illuminations = spec->illuminations.data();
   const std::size_t image = static_cast<std::size_t>(o.hologram_size) * o.hologram_size;
   spec->backgrounds.assign(backgrounds, backgrounds + image * o.count);
-  return add_kind([spec] { return std::make_unique<ReconWorker>(*spec); });
+  return add_kind(kReconstruction, [spec] { return std::make_unique<ReconWorker>(*spec); });
 }
 
 int Server::add_bpm(const BpmInversionSpec& spec) {
@@ -104,16 +104,17 @@ int Server::add_bpm(const BpmInversionSpec& spec) {
   // An automatic illumination chunk is chosen from the free memory when the first slot is
   // built (inside add_kind, before the geometry is visible to the dispatcher) and kept for every
   // later slot, so that all slots have the footprint admission measured on the first.
-  return add_kind([shared] {
+  return add_kind(kBpm, [shared] {
     auto w = std::make_unique<BpmWorker>(*shared);
     if (shared->chunk == 0) shared->chunk = w->chunk();
     return w;
   });
 }
 
-int Server::add_kind(std::function<std::unique_ptr<SlotWorker>()> make) {
+int Server::add_kind(Kind kind, std::function<std::unique_ptr<SlotWorker>()> make) {
   auto g = std::make_unique<Geometry>();
   g->make = std::move(make);
+  g->kind = kind;
   // The first slot is built now: it measures the footprint and fails early when it does not fit.
   // The lock keeps the dispatcher from building slots meanwhile (it builds them under the same
   // lock), so the change in free memory is this slot's alone; running slots allocate nothing.
@@ -141,7 +142,8 @@ int Server::add_kind(std::function<std::unique_ptr<SlotWorker>()> make) {
   return id;
 }
 
-std::int64_t Server::submit(int geometry, const void* input, float* ri_out, double* extra) {
+std::int64_t Server::submit(int geometry, Kind kind, const void* input, float* ri_out,
+                            double* extra) {
   require(input != nullptr && ri_out != nullptr, "Server::submit: null buffer");
   auto r = std::make_shared<Request>();
   r->input = input;
@@ -152,6 +154,9 @@ std::int64_t Server::submit(int geometry, const void* input, float* ri_out, doub
     std::lock_guard<std::mutex> lock(mu_);
     require(geometry >= 0 && geometry < static_cast<int>(geometries_.size()),
             "Server::submit: unknown geometry");
+    require(geometries_[geometry]->kind == kind,
+            kind == kBpm ? "Server::submit_bpm: not a BPM geometry"
+                         : "Server::submit: a BPM geometry takes submit_bpm");
     r->geometry = geometry;
     r->id = next_id_++;
     if (submitted_ == 0) first_arrival_ = r->arrival;

@@ -53,7 +53,11 @@ class Server {
   // ratios (count, n, n) complex of a BPM one), ri_out (nz, n, n) and extra (the per-iteration
   // loss of a BPM request, may be null) are host memory that must stay valid until wait()
   // returns for this id.
-  std::int64_t submit(int geometry, const void* input, float* ri_out, double* extra = nullptr);
+  // kind: the request kind the caller's input is for (kReconstruction or kBpm); a geometry of
+  // the other kind is refused.
+  enum Kind { kReconstruction = 0, kBpm = 1 };
+  std::int64_t submit(int geometry, Kind kind, const void* input, float* ri_out,
+                      double* extra = nullptr);
   // Blocks until the request is done; returns its status and message.
   int wait(std::int64_t id, double* filled, double* latency_ms, std::string* error);
   bool done(std::int64_t id);
@@ -63,6 +67,7 @@ class Server {
  private:
   struct Geometry {
     std::function<std::unique_ptr<SlotWorker>()> make;
+    Kind kind = kReconstruction;
     std::size_t slot_bytes = 0;  // device bytes of one slot, measured on the first
     int slots = 0;
   };
@@ -90,7 +95,7 @@ class Server {
     std::thread thread;
   };
 
-  int add_kind(std::function<std::unique_ptr<SlotWorker>()> make);
+  int add_kind(Kind kind, std::function<std::unique_ptr<SlotWorker>()> make);
   void dispatch_loop();
   void slot_loop(Slot* slot);
   bool try_admit(const std::shared_ptr<Request>& r);

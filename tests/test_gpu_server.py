@@ -161,3 +161,26 @@ def test_bpm_geometry_over_budget_is_refused(device, qh, bpm_mixed):
     req, kw, _, _ = bpm_mixed[0]
     with pytest.raises(qh.OutOfDeviceMemory):
         server.add_bpm(req.fields.shape[1], req.illuminations, req.geometry, **kw)
+
+
+def test_c_abi_refuses_a_request_of_the_other_kind(device, qh, mixed, bpm_mixed):
+    """The C entry points check the geometry's request kind (the Python layer checks first)."""
+    import ctypes
+
+    from qoraholo_amd import _native
+
+    server = qh.Server(max_slots=2)
+    req, kw, _, _ = bpm_mixed[0]
+    bpm = server.add_bpm(req.fields.shape[1], req.illuminations, req.geometry, **kw)
+    rec = add_all(server, mixed[:1])[0]
+    out = np.empty((64, 64, 64), np.float32)
+    rid = ctypes.c_int64()
+    holo = mixed[0][0].holo
+    with pytest.raises(ValueError):
+        _native.check(_native.lib().qh_hip_server_submit(server._handle, bpm, _native.ptr(holo),
+                                                         _native.ptr(out), ctypes.byref(rid)))
+    with pytest.raises(ValueError):
+        _native.check(_native.lib().qh_hip_server_submit_bpm(
+            server._handle, rec, _native.ptr(req.fields), _native.ptr(out), None,
+            ctypes.byref(rid)))
+    assert server.stats()["submitted"] == 0
