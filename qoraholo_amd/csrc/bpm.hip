@@ -333,12 +333,14 @@ void launch_with(Kind k, Args a, unsigned grid, std::size_t lds, hipStream_t s) 
 
 // Rows: blocks of 8 lines (2 waves at 256; 4 or 16 lines were no faster). Columns: at least
 // 16 lines per block, so that each block row is a whole 128-byte segment (blocks of 8 lines
-// would share every cache line with a block that may run on another XCD).
+// would share every cache line with a block that may run on another XCD); 32 at 256 and 384,
+// where that was faster (384: forward 62.7 -> 59.5 ms; 192 was slower with 32).
 template <bool ROWS, int N1, int N2>
 void launch_reg(Kind k, Args a, hipStream_t s) {
   constexpr int kT = N1 > N2 ? N1 : N2;
   constexpr int kRows = 8 * kT;
-  constexpr int kCols = 16 * kT > 256 ? 16 * kT : 256;
+  constexpr int kCols = (N1 * N2 == 256 || N1 * N2 == 384) ? 32 * kT
+                                                             : (16 * kT > 256 ? 16 * kT : 256);
   using E = RegLines<N1, N2, !ROWS, ROWS ? kRows : kCols>;
   a.g.lines_per_block = E::LINES;
   launch_with<ROWS, E>(k, a, E::grid(a.g.lines), 0, s);
