@@ -287,8 +287,8 @@ def served_mixed(qh, rates=(0.25, 0.5, 1.0, 2.0), per_rate=16, seed=0):
     return out
 
 
-def served_bpm(qh, sizes=((192, 49), (384, 81)), iterations=40, seed=0, repeats=(4, 2),
-               max_slots=4):
+def served_bpm(qh, sizes=((192, 49), (384, 81)), iterations=40, seed=0, repeats=None,
+               max_slots=4, stream=True):
     """Config 5: BPM inversion requests from host arrays. Each request is the reference's recipe
     (examples/bpm_figure.py:74-88): Rytov of the field ratios, 20 GP iterations for the start,
     then 40 full-batch FISTA iterations of the BPM with the TV prox (weight 1e-2, 10 inner
@@ -297,8 +297,9 @@ def served_bpm(qh, sizes=((192, 49), (384, 81)), iterations=40, seed=0, repeats=
     the per-iteration time; per iteration, the reference's traffic model of a gradient --
     forward, 15/16 recomputation from checkpoints, adjoint -- is the algorithmic figure); then
     the same requests through the Server, its BPM geometries resident and admitted by their
-    measured HBM footprint: a burst of `repeats[k]` requests of size k, interleaved, with
-    inversions/s, p50/p99 latency and each result compared with the per-call one."""
+    measured HBM footprint: a burst of `repeats[n]` requests of side n (4 of 192, 2 of 384),
+    interleaved, with inversions/s, p50/p99 latency and each result compared with the per-call
+    one."""
     from qoraholo_amd import workloads
 
     out = []
@@ -346,6 +347,8 @@ def served_bpm(qh, sizes=((192, 49), (384, 81)), iterations=40, seed=0, repeats=
         log(f"served bpm {n}^3 x {a}: {row}")
         out.append(row)
         reqs.append((req, ri, loss))
+    if not stream:
+        return {"requests": out}
 
     server = qh.Server(max_slots=max_slots)
     gids = [server.add_bpm(n, req.illuminations, req.geometry, start_iterations=ITERATIONS,
@@ -355,8 +358,10 @@ def served_bpm(qh, sizes=((192, 49), (384, 81)), iterations=40, seed=0, repeats=
         server.wait(server.submit_bpm(gid, reqs[k][0].fields))
     server.reset_stats()
     order = []
-    for i in range(max(repeats[:len(sizes)])):
-        order += [k for k in range(len(sizes)) if i < repeats[k]]
+    repeats = repeats or {192: 4, 384: 2}
+    counts = [repeats.get(n, 2) for n, _ in sizes]
+    for i in range(max(counts)):
+        order += [k for k in range(len(sizes)) if i < counts[k]]
     log(f"served bpm stream: {len(order)} requests, sizes {[sizes[k][0] for k in order]}")
     t0 = time.perf_counter()
     ids = [(k, server.submit_bpm(gids[k], reqs[k][0].fields)) for k in order]
@@ -493,6 +498,8 @@ def main() -> int:
                     help="only the served BPM inversion requests (192^3 x 49, 384^3 x 81)")
     ap.add_argument("--served-bpm-size", type=int, choices=(192, 384), default=0,
                     help="with --served-bpm: only the request of this size")
+    ap.add_argument("--no-served-stream", action="store_true",
+                    help="with --served-bpm: only the per-call requests, not the served stream")
     ap.add_argument("--fgp-traffic", action="store_true",
                     help="run only 5 FGP steps at 256^3 and a 256 MiB device copy, for the "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE runs that calibrate HBM traffic")
@@ -520,7 +527,7 @@ def main() -> int:
         sizes = ((192, 49), (384, 81))
         if args.served_bpm_size:
             sizes = tuple(v for v in sizes if v[0] == args.served_bpm_size)
-        print(json.dumps(served_bpm(qh, sizes=sizes)), flush=True)
+        print(json.dumps(served_bpm(qh, sizes=sizes, stream=not args.no_served_stream)), flush=True)
         return 0
     if args.bpm_only:
         print(json.dumps(bpm_leg(qh, torch, cpu=not args.no_cpu_baseline, repeats=args.steps)),
