@@ -127,19 +127,50 @@ int Server::add_kind(Kind kind, std::function<std::unique_ptr<SlotWorker>()> mak
   const std::size_t after = free_device_bytes();
   slot->bytes = before > after ? before - after : 0;
   g->slot_bytes = std::max<std::size_t>(slot->bytes, 1);
-  if (used_bytes_ + slot->bytes > budget_) {
+  if (slot->bytes > budget_) {
     throw OutOfMemory("Server: one slot of this geometry needs " + std::to_string(slot->bytes) +
                       " bytes, over the HBM budget");
   }
   const int id = static_cast<int>(geometries_.size());
+  // Idle slots of other geometries make room as they do for a request; when the slot still does
+  // not fit (the others are busy), the geometry is registered without a resident slot and its
+  // first request builds one.
+  while (used_bytes_ + slot->bytes > budget_ || live_slots() >= max_slots_) {
+    if (!evict_idle(id)) break;
+  }
+  geometries_.push_back(std::move(g));
+  if (used_bytes_ + slot->bytes > budget_ || live_slots() >= max_slots_) return id;
   slot->geometry = id;
-  g->slots = 1;
+  geometries_[id]->slots = 1;
   used_bytes_ += slot->bytes;
   Slot* raw = slot.get();
   slots_.push_back(std::move(slot));
   raw->thread = std::thread([this, raw] { slot_loop(raw); });
-  geometries_.push_back(std::move(g));
   return id;
+}
+
+int Server::live_slots() const {
+  int n = 0;
+  for (const auto& s : slots_) n += s->pipeline ? 1 : 0;
+  return n;
+}
+
+bool Server::evict_idle(int keep) {
+  auto it = std::find_if(slots_.begin(), slots_.end(), [&](const std::unique_ptr<Slot>& s) {
+    return s->pipeline && !s->busy && s->geometry != keep;
+  });
+  if (it == slots_.end()) return false;
+  // The victim is idle: free its device memory now; its thread leaves its wait once the lock is
+  // released and is joined with the retired slots.
+  Slot* victim = it->get();
+  victim->pipeline.reset();
+  victim->stop = true;
+  victim->cv.notify_all();
+  used_bytes_ -= std::min(used_bytes_, victim->bytes);
+  --geometries_[victim->geometry]->slots;
+  retired_.push_back(std::move(*it));
+  slots_.erase(it);
+  return true;
 }
 
 std::int64_t Server::submit(int geometry, Kind kind, const void* input, float* ri_out,
@@ -180,27 +211,9 @@ bool Server::try_admit(const std::shared_ptr<Request>& r) {
     }
   }
   Geometry& g = *geometries_[r->geometry];
-  const auto live = [&] {
-    int n = 0;
-    for (auto& s : slots_) n += s->pipeline ? 1 : 0;
-    return n;
-  };
-  while ((used_bytes_ + g.slot_bytes > budget_ || live() >= max_slots_)) {
+  while (used_bytes_ + g.slot_bytes > budget_ || live_slots() >= max_slots_) {
     // Evict one idle slot of another geometry; stop when none is idle.
-    auto it = std::find_if(slots_.begin(), slots_.end(), [&](const std::unique_ptr<Slot>& s) {
-      return s->pipeline && !s->busy && s->geometry != r->geometry;
-    });
-    if (it == slots_.end()) return false;
-    // The victim is idle: free its device memory now; its thread leaves its wait once the lock
-    // is released and is joined with the retired slots.
-    Slot* victim = it->get();
-    victim->pipeline.reset();
-    victim->stop = true;
-    victim->cv.notify_all();
-    used_bytes_ -= std::min(used_bytes_, victim->bytes);
-    --geometries_[victim->geometry]->slots;
-    retired_.push_back(std::move(*it));
-    slots_.erase(it);
+    if (!evict_idle(r->geometry)) return false;
   }
   auto slot = std::make_unique<Slot>();
   slot->geometry = r->geometry;

@@ -96,6 +96,10 @@ class Server {
   };
 
   int add_kind(Kind kind, std::function<std::unique_ptr<SlotWorker>()> make);
+  // Called with mu_ held: slots holding device state, and the eviction of one idle slot of a
+  // geometry other than `keep` (false when there is none).
+  int live_slots() const;
+  bool evict_idle(int keep);
   void dispatch_loop();
   void slot_loop(Slot* slot);
   bool try_admit(const std::shared_ptr<Request>& r);

@@ -184,3 +184,30 @@ def test_c_abi_refuses_a_request_of_the_other_kind(device, qh, mixed, bpm_mixed)
             server._handle, rec, _native.ptr(req.fields), _native.ptr(out), None,
             ctypes.byref(rid)))
     assert server.stats()["submitted"] == 0
+
+
+def test_tight_budget_evicts_across_request_kinds(device, qh, mixed, bpm_mixed):
+    """With room for one slot at a time, a reconstruction geometry and a BPM geometry take turns:
+    registering the second evicts the first's idle slot, and every request evicts the other
+    kind's; each result is still the one the request gets alone."""
+    probe = qh.Server(max_slots=4)
+    req, kw, ri_bpm, loss_bpm = bpm_mixed[0]
+    probe.add_bpm(req.fields.shape[1], req.illuminations, req.geometry, **kw)
+    bpm_bytes = probe.stats()["device_bytes"]
+    add_all(probe, mixed[:1])
+    rec_bytes = probe.stats()["device_bytes"] - bpm_bytes
+    del probe
+    budget = max(bpm_bytes, rec_bytes) + min(bpm_bytes, rec_bytes) // 2  # one slot at a time
+    server = qh.Server(budget_bytes=budget, max_slots=4)
+    bpm = server.add_bpm(req.fields.shape[1], req.illuminations, req.geometry, **kw)
+    rec = add_all(server, mixed[:1])[0]
+    assert server.stats()["slots"] == 1
+    for kind in ("r", "b", "r", "b", "b", "r"):
+        if kind == "b":
+            ri, loss, _ = server.wait(server.submit_bpm(bpm, req.fields))
+            assert np.array_equal(ri, ri_bpm) and np.array_equal(loss, loss_bpm)
+        else:
+            ri, _, _ = server.wait(server.submit(rec, mixed[0][0].holo))
+            assert np.array_equal(ri, mixed[0][1])
+        s = server.stats()
+        assert s["device_bytes"] <= budget and s["slots"] == 1
