@@ -215,10 +215,8 @@ def served_mixed(qh, rates=(0.25, 0.5, 1.0, 2.0), per_rate=40, seed=0, max_slots
     and 81 x 768^2 -> 384^3 (random cells of 3-8 nested or disjoint spheres), solvers uniform
     over GP 20 and FISTA-TV 20 x 10. Capacity = 1 / mean service time of the plain per-call
     chain over the six kinds (median of 3 timed requests each); also reported: serial
-    Pipeline.run per kind. Eight slots: with fewer than the six kinds, requests of a kind that
-    has no resident slot wait for another kind's idle slot to be evicted and a new one built
-    (2, 3, 4, 6, 8 and 12 slots served 24.0, 25.3, 24.4, 28.6, 29.3 and 31.7 requests/s at twice
-    the plain capacity; p99 grows past 8)."""
+    Pipeline.run per kind. Up to eight requests in flight (four served as many requests per
+    second at twice the plain capacity, with a longer queue); idle slots stay resident."""
     from qoraholo_amd import workloads
 
     kinds = []

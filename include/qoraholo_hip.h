@@ -258,11 +258,13 @@ typedef struct {
   double recon_per_s; /* completed / (last completion - first submission) */
   size_t device_bytes;
   size_t budget_bytes;
-  int slots;
-  int busy_slots;
+  int slots;           /* slots resident on the device (idle ones keep their geometry's state) */
+  int busy_slots;      /* slots running a request now */
+  int peak_busy_slots; /* most slots running requests at once since the last reset */
 } qh_server_stats;
 
-/* budget_bytes 0: 90% of the free device memory. max_slots: requests in flight at once. */
+/* budget_bytes 0: 90% of the free device memory. max_slots: requests in flight at once; idle
+ * slots stay resident until the budget needs their memory. */
 int qh_hip_server_create(size_t budget_bytes, int max_slots, qh_server** out);
 /* Finishes the requests on the device, drops the queued ones, frees everything. */
 void qh_hip_server_destroy(qh_server* s);

@@ -62,7 +62,7 @@ class ServerStats_t(ctypes.Structure):
                 ("failed", ctypes.c_int64), ("queued", ctypes.c_int64), ("p50_ms", c_double),
                 ("p99_ms", c_double), ("mean_ms", c_double), ("recon_per_s", c_double),
                 ("device_bytes", c_size_t), ("budget_bytes", c_size_t), ("slots", c_int),
-                ("busy_slots", c_int)]
+                ("busy_slots", c_int), ("peak_busy_slots", c_int)]
 
 
 class BpmOptions_t(ctypes.Structure):

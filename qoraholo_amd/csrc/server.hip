@@ -85,6 +85,7 @@ void Server::reset_stats() {
   std::lock_guard<std::mutex> lock(mu_);
   latencies_.clear();
   submitted_ = completed_ = failed_ = 0;
+  peak_busy_ = busy_slots();
 }
 
 int Server::add_geometry(const qh_pipeline_options& o, const float* backgrounds) {
@@ -135,11 +136,11 @@ int Server::add_kind(Kind kind, std::function<std::unique_ptr<SlotWorker>()> mak
   // Idle slots of other geometries make room as they do for a request; when the slot still does
   // not fit (the others are busy), the geometry is registered without a resident slot and its
   // first request builds one.
-  while (used_bytes_ + slot->bytes > budget_ || live_slots() >= max_slots_) {
+  while (used_bytes_ + slot->bytes > budget_) {
     if (!evict_idle(id)) break;
   }
   geometries_.push_back(std::move(g));
-  if (used_bytes_ + slot->bytes > budget_ || live_slots() >= max_slots_) return id;
+  if (used_bytes_ + slot->bytes > budget_) return id;
   slot->geometry = id;
   geometries_[id]->slots = 1;
   used_bytes_ += slot->bytes;
@@ -149,9 +150,9 @@ int Server::add_kind(Kind kind, std::function<std::unique_ptr<SlotWorker>()> mak
   return id;
 }
 
-int Server::live_slots() const {
+int Server::busy_slots() const {
   int n = 0;
-  for (const auto& s : slots_) n += s->pipeline ? 1 : 0;
+  for (const auto& s : slots_) n += s->busy ? 1 : 0;
   return n;
 }
 
@@ -202,16 +203,20 @@ std::int64_t Server::submit(int geometry, Kind kind, const void* input, float* r
 // Called with mu_ held: an idle slot of the request's geometry, or a new slot when the budget
 // and the slot limit allow one (evicting idle slots of other geometries if needed).
 bool Server::try_admit(const std::shared_ptr<Request>& r) {
+  // max_slots bounds the requests in flight; idle slots stay resident (background fields, plans
+  // and solver state of their geometry) until the HBM budget needs their memory.
+  if (busy_slots() >= max_slots_) return false;
   for (auto& s : slots_) {
     if (!s->busy && !s->stop && s->geometry == r->geometry && s->pipeline) {
       s->busy = true;
+      peak_busy_ = std::max(peak_busy_, busy_slots());
       s->work = r;
       s->cv.notify_all();
       return true;
     }
   }
   Geometry& g = *geometries_[r->geometry];
-  while (used_bytes_ + g.slot_bytes > budget_ || live_slots() >= max_slots_) {
+  while (used_bytes_ + g.slot_bytes > budget_) {
     // Evict one idle slot of another geometry; stop when none is idle.
     if (!evict_idle(r->geometry)) return false;
   }
@@ -225,6 +230,7 @@ bool Server::try_admit(const std::shared_ptr<Request>& r) {
   slot->work = r;
   Slot* raw = slot.get();
   slots_.push_back(std::move(slot));
+  peak_busy_ = std::max(peak_busy_, busy_slots());
   raw->thread = std::thread([this, raw] { slot_loop(raw); });
   return true;
 }
@@ -354,6 +360,7 @@ qh_server_stats Server::stats() {
   }
   s.slots = live;
   s.busy_slots = busy;
+  s.peak_busy_slots = peak_busy_;
   return s;
 }
 

@@ -96,9 +96,9 @@ class Server {
   };
 
   int add_kind(Kind kind, std::function<std::unique_ptr<SlotWorker>()> make);
-  // Called with mu_ held: slots holding device state, and the eviction of one idle slot of a
+  // Called with mu_ held: slots running a request, and the eviction of one idle slot of a
   // geometry other than `keep` (false when there is none).
-  int live_slots() const;
+  int busy_slots() const;
   bool evict_idle(int keep);
   void dispatch_loop();
   void slot_loop(Slot* slot);
@@ -115,6 +115,7 @@ class Server {
   std::vector<std::unique_ptr<Slot>> slots_;
   std::vector<std::unique_ptr<Slot>> retired_;
   std::size_t used_bytes_ = 0;
+  int peak_busy_ = 0;  // most slots running requests at once since the last reset_stats
   std::int64_t next_id_ = 1;
   bool stop_ = false;
   std::vector<double> latencies_;

@@ -43,7 +43,9 @@ def test_results_are_the_same_alone_and_in_the_mix(device, qh, mixed, max_slots)
     s = server.stats()
     assert s["completed"] == len(order) and s["failed"] == 0
     assert 0 < s["p50_ms"] <= s["p99_ms"]
-    assert s["slots"] <= max_slots
+    assert 1 <= s["peak_busy_slots"] <= max_slots
+    # Idle slots stay resident: each of the three geometries keeps one (the budget is ample).
+    assert s["slots"] >= len(mixed)
 
 
 def test_submissions_from_several_threads(device, qh, mixed):
@@ -138,7 +140,7 @@ def test_bpm_requests_equal_the_per_call_chain_alone_and_in_the_mix(device, qh, 
             assert np.array_equal(out, mixed[k][1]), (kind, k)
     s = server.stats()
     assert s["completed"] == len(order) + len(bids) and s["failed"] == 0
-    assert s["slots"] <= 4
+    assert s["peak_busy_slots"] <= 4
 
 
 def test_bpm_requests_are_checked(device, qh, bpm_mixed):
