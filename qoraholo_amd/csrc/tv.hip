@@ -524,10 +524,11 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
   __syncthreads();
   lds_get<VX, VEC>(zk, wd, LW, xl, zdn_k);  // z_k of the row below on plane zs
 
-  // The loads of the next plane are issued right after the second barrier, so that their
-  // latency overlaps the second primal and the last dual step instead of stalling the block at
-  // the next barrier (issued after the third barrier, the pass took 4% longer; after the first,
-  // the registers spill); r_k is formed from them at the top of the next iteration.
+  // The loads of the next plane are issued once the first dual step is done, before the second
+  // barrier, so that their latency overlaps the second primal and the last dual step instead
+  // of stalling the block at the next barrier (issued after the third barrier the pass took 6%
+  // longer at 256 and 7% at 384; any earlier, the registers spill); r_k is formed from them at
+  // the top of the next iteration.
   // What each wave's row feeds: the output rows (waves 2 .. NW-3) need p_{k+2}, which needs
   // z_{k+1} one row further down (waves 2 .. NW-2), which needs r_{k+1} one row up (waves
   // 1 .. NW-2), which needs z_k one row down (waves 1 .. NW-1); wave 0 only supplies r_k. A
@@ -566,11 +567,11 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
         for (int v = 0; v < VX; ++v) rq1[c][v] = pq1[c][v] + a.m2 * (pq1[c][v] - pk1[c][v]);
       }
     }
+    float pk3[3][VX], pm3[3][VX], b3[VX];
+    load_raw(f + 1, pk3, pm3, b3);
     lds_put<VX, VEC>(zk, w, LW, xl, zk2);
     lds_put<VX, VEC>(rqy, w, LW, xl, rq1[1]);
     __syncthreads();  // (2) z_k of plane f and r_{k+1} of plane t + 1 visible
-    float pk3[3][VX], pm3[3][VX], b3[VX];
-    load_raw(f + 1, pk3, pm3, b3);
     lds_get<VX, VEC>(zk, wd, LW, xl, zdn_k);  // for the next plane's dual step
     float zq1[VX] = {};
     if (need_zq) {
