@@ -225,7 +225,9 @@ int qh_hip_pipeline_run(qh_pipeline* p, const float* holograms, float* ri);
 /* The same from holograms already in device memory into a device volume, on `stream`
  * (a hipStream_t, used as given: NULL is the null stream, which is also torch's default
  * stream). Returns once the work is queued; the caller synchronises the stream before reading
- * ri_device. */
+ * ri_device. Requests share the pipeline's device buffers, so the pipeline orders its own
+ * work: a later process_device (on any stream), process, run or set_background starts after
+ * this request's work; the caller need not synchronise between calls. */
 int qh_hip_pipeline_process_device(qh_pipeline* p, const float* holograms_device,
                                    float* ri_device, void* stream);
 double qh_hip_pipeline_filled(const qh_pipeline* p);

@@ -45,6 +45,15 @@ def lib():
     return _lib
 
 
+def set_threads(threads: int) -> None:
+    """OpenMP threads of later oracle calls."""
+    lib().oqh_set_threads(int(threads))
+
+
+def max_threads() -> int:
+    return int(lib().oqh_max_threads())
+
+
 def _ptr(a: np.ndarray):
     return a.ctypes.data_as(ctypes.c_void_p)
 

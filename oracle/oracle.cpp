@@ -17,6 +17,8 @@
 #include <string>
 #include <vector>
 
+#include <omp.h>
+
 #include "fft.hpp"
 #include "shared.hpp"
 
@@ -557,6 +559,10 @@ const oqh::cx<T>* as_cx(const T* p) { return reinterpret_cast<const oqh::cx<T>*>
 extern "C" {
 
 const char* oqh_last_error() { return g_error.c_str(); }
+
+// OpenMP threads of later calls (the CPU baseline times one thread and all of them).
+void oqh_set_threads(int threads) { omp_set_num_threads(threads > 0 ? threads : 1); }
+int oqh_max_threads() { return omp_get_max_threads(); }
 
 int oqh_fft_c2c_f(float* d, int d0, int d1, int d2, unsigned axes, int forward, float scale) {
   return guarded([&] { oqh::fft3<float>(as_cx(d), d0, d1, d2, axes, forward != 0, scale); });

@@ -22,6 +22,7 @@
 #include "runtime.hpp"
 #include "server.hpp"
 #include "stages.hpp"
+#include "status.hpp"
 
 struct qh_pipeline {
   std::unique_ptr<qh::Pipeline> impl;
@@ -37,33 +38,12 @@ struct qh_bpm_plan {
 
 namespace qh {
 
-namespace {
-
-thread_local std::string g_error;
-
-template <class F>
-int guarded(F&& f) {
-  try {
-    f();
-    g_error.clear();
-    return QH_OK;
-  } catch (const InvalidArgument& e) {
-    g_error = e.what();
-    return QH_ERR_INVALID_ARGUMENT;
-  } catch (const LogicError& e) {
-    g_error = e.what();
-    return QH_ERR_LOGIC;
-  } catch (const OutOfMemory& e) {
-    g_error = e.what();
-    return QH_ERR_OUT_OF_MEMORY;
-  } catch (const std::bad_alloc&) {
-    g_error = "out of host memory";
-    return QH_ERR_OUT_OF_MEMORY;
-  } catch (const std::exception& e) {
-    g_error = e.what();
-    return QH_ERR_RUNTIME;
-  }
+std::string& last_error() {
+  thread_local std::string message;
+  return message;
 }
+
+namespace {
 
 // Per-thread stream of the per-call entry points, so calls from different host threads run
 // independently.
@@ -253,7 +233,7 @@ using namespace qh;
 
 extern "C" {
 
-const char* qh_hip_last_error(void) { return g_error.c_str(); }
+const char* qh_hip_last_error(void) { return last_error().c_str(); }
 
 int qh_hip_device_present(void) {
   int n = 0;

@@ -63,8 +63,13 @@ Pipeline::Pipeline(const qh_pipeline_options& o)
   *filled_.get() = 0;
 }
 
+void Pipeline::after_device_work(hipStream_t s) {
+  if (device_pending_) check(hipStreamWaitEvent(s, device_done_.get(), 0), "hipStreamWaitEvent");
+}
+
 void Pipeline::set_background(const float* host) {
   const hipStream_t s = compute_.get();
+  after_device_work(s);
   const std::size_t image = static_cast<std::size_t>(n_) * n_;
   check(hipMemcpyAsync(holo_.get(), host, count_ * image * sizeof(float), hipMemcpyHostToDevice, s),
         "upload backgrounds");
@@ -72,6 +77,7 @@ void Pipeline::set_background(const float* host) {
   crop_fields(half_.get(), count_, n_, crop_, background_.get(), s);
   fields_inverse(background_.get(), count_, m_, 1.0f, false, s);
   check(hipStreamSynchronize(s), "set_background");
+  device_pending_ = false;
   has_background_ = true;
 }
 
@@ -124,6 +130,10 @@ void Pipeline::process_host(const float* user_in, float* user_out) {
   if (!has_background_) throw LogicError("Pipeline: set_background() first");
   const auto t0 = std::chrono::steady_clock::now();
   const hipStream_t s = compute_.get();
+  // Both streams start after any request still queued on a caller's stream (the copy stream
+  // below waits on events recorded on s after this point).
+  after_device_work(s);
+  after_device_work(copy_.get());
   if (use_graph_ && !solver_.timing_fgp()) {
     // Uploads into holo_ on the copy stream once the previous request's work on s is done,
     // then the captured front end and solver, which read holo_.
@@ -174,6 +184,7 @@ void Pipeline::process_host(const float* user_in, float* user_out) {
     }
   }
   check(hipStreamSynchronize(s), "process");
+  device_pending_ = false;
   collect(true);
   timings_pending_ = false;
   timings_.upload = graph_ran_ ? elapsed_ms(upload_start_, chunk_done_.back())
@@ -227,6 +238,7 @@ void Pipeline::compute(const float* holo_dev, hipStream_t s) {
 
 void Pipeline::process_device(const float* holo_dev, float* ri_dev, hipStream_t s) {
   if (!has_background_) throw LogicError("Pipeline: set_background() first");
+  after_device_work(s);
   compute(holo_dev, s);
   const std::size_t vol = static_cast<std::size_t>(solver_.nz()) * solver_.n() * solver_.n();
   check(hipMemcpyAsync(ri_dev, solver_.index(), vol * sizeof(float), hipMemcpyDeviceToDevice, s),
@@ -235,6 +247,8 @@ void Pipeline::process_device(const float* holo_dev, float* ri_dev, hipStream_t 
                        hipMemcpyDeviceToHost, s),
         "download filled");
   clock_.mark(kDownloaded, s);
+  device_done_.record(s);
+  device_pending_ = true;
   timings_pending_ = true;
 }
 

@@ -57,6 +57,9 @@ class Pipeline {
   // captured for holo_dev (captured now if there is none), or kernel by kernel.
   void compute(const float* holo_dev, hipStream_t s);
   void collect(bool downloaded);
+  // Orders stream s after the work the last process_device() queued on its caller's stream:
+  // every request shares the pipeline's buffers, whatever stream it runs on.
+  void after_device_work(hipStream_t s);
 
   int count_, n_, m_, chunk_, chunks_;
   CropPlan crop_;
@@ -76,6 +79,8 @@ class Pipeline {
   std::vector<Event> chunk_done_;
   std::vector<Event> out_done_;
   Event upload_start_;
+  Event device_done_;  // recorded after the last process_device() on its caller's stream
+  bool device_pending_ = false;
   GraphExec graph_;
   const float* graph_input_ = nullptr;
   bool use_graph_ = true;

@@ -817,19 +817,38 @@ void fgp_pair(const float* b, const float* p_k, const float* p_km1, float* p_k1,
   check_launch("fgp_pair");
 }
 
+namespace {
+
+// Floats of one dual buffer (three components), rounded up to 256 bytes so that every buffer
+// of a workspace starts 256-byte aligned.
+std::size_t dual_floats(int nz, int ny, int nx) {
+  const std::size_t three = 3 * static_cast<std::size_t>(nz) * ny * nx;
+  return (three + 63) / 64 * 64;
+}
+
+}  // namespace
+
+std::size_t TvState::workspace_floats(int nz, int ny, int nx) { return 4 * dual_floats(nz, ny, nx); }
+
 TvState::TvState(int nz, int ny, int nx)
     : nz_(nz), ny_(ny), nx_(nx), vol_(static_cast<long>(nz) * ny * nx),
-      p_{DeviceBuffer<float>(3 * vol_), DeviceBuffer<float>(3 * vol_),
-         DeviceBuffer<float>(3 * vol_), DeviceBuffer<float>(3 * vol_)} {}
+      owned_(workspace_floats(nz, ny, nx)) {
+  const std::size_t stride = dual_floats(nz, ny, nx);
+  for (int i = 0; i < 4; ++i) p_[i] = owned_.get() + i * stride;
+}
+
+TvState::TvState(int nz, int ny, int nx, float* workspace)
+    : nz_(nz), ny_(ny), nx_(nx), vol_(static_cast<long>(nz) * ny * nx) {
+  const std::size_t stride = dual_floats(nz, ny, nx);
+  for (int i = 0; i < 4; ++i) p_[i] = workspace + i * stride;
+}
 
 void TvState::reset(hipStream_t s) {
-  fill(p_[0].get(), 3 * vol_, 0.0f, s);
+  fill(p_[0], 3 * vol_, 0.0f, s);
   cur_ = 0;
 }
 
-std::size_t TvState::bytes() const {
-  return p_[0].bytes() + p_[1].bytes() + p_[2].bytes() + p_[3].bytes();
-}
+std::size_t TvState::bytes() const { return owned_.bytes(); }
 
 // tv_prox of the reference (src/cpu/tv.cpp:7-44): r <- p, `inner` FGP steps, out = primal(p).
 // The dual iterates rotate through four buffers: a pass reads p_k and p_{k-1} (r is formed
@@ -870,7 +889,7 @@ void TvState::prox(const float* b, float* out, float lambda, int inner, hipStrea
       t = tn;
       // The last pass skips p_{k+1}: the next prox starts from p alone.
       const bool last = k + 2 == inner;
-      fgp_pair(b, p_[cur].get(), p_[prev].get(), last ? nullptr : p_[f1].get(), p_[f2].get(),
+      fgp_pair(b, p_[cur], p_[prev], last ? nullptr : p_[f1], p_[f2],
                nz_, ny_, nx_, lambda, step, m1, k == 0, m2, s);
       tn = fista_t(t);
       m_prev = (t - 1.0f) / tn;
@@ -879,7 +898,7 @@ void TvState::prox(const float* b, float* out, float lambda, int inner, hipStrea
       cur = f2;
       k += 2;
     } else {
-      fgp_step(b, p_[cur].get(), p_[prev].get(), p_[f1].get(), nz_, ny_, nx_, lambda, step,
+      fgp_step(b, p_[cur], p_[prev], p_[f1], nz_, ny_, nx_, lambda, step,
                m_prev, k == 0, s);
       const float tn = fista_t(t);
       m_prev = (t - 1.0f) / tn;
@@ -892,7 +911,7 @@ void TvState::prox(const float* b, float* out, float lambda, int inner, hipStrea
   }
   if (t1) t1->record(s);
   cur_ = cur;
-  tv_primal(b, p_[cur_].get(), nz_, ny_, nx_, lambda, out, to_index, nm2, inv_k02, s);
+  tv_primal(b, p_[cur_], nz_, ny_, nx_, lambda, out, to_index, nm2, inv_k02, s);
 }
 
 }  // namespace qh

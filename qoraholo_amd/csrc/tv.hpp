@@ -36,7 +36,12 @@ void tv_primal(const float* b, const float* p, int nz, int ny, int nx, float lam
 // runs them one per pass, for comparisons).
 class TvState {
  public:
+  // The dual buffers in device memory of its own.
   TvState(int nz, int ny, int nx);
+  // The dual buffers in `workspace` (workspace_floats(nz, ny, nx) floats of device memory,
+  // 256-byte aligned, not owned: it must outlive the state).
+  TvState(int nz, int ny, int nx, float* workspace);
+  static std::size_t workspace_floats(int nz, int ny, int nx);
   void reset(hipStream_t s);
   // out = prox_{lambda TV, x >= 0}(b) by `inner` FGP steps from the current dual.
   // Events t0 and t1, when given, bracket the FGP steps (not the final primal kernel).
@@ -51,7 +56,8 @@ class TvState {
  private:
   int nz_, ny_, nx_;
   long vol_;
-  DeviceBuffer<float> p_[4];
+  DeviceBuffer<float> owned_;
+  float* p_[4];  // dual buffers of three components each, inside owned_ or the workspace
   int cur_ = 0;
   bool fused_ = true;
   int launches_ = 0;

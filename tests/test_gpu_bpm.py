@@ -3,6 +3,8 @@ checkpointing, gradient and loss, the inversion, the torch.autograd.Function, an
 at the full 256^2 x 256 / 49-illumination shape (unit envelope, checkpoint and chunk
 invariance bit for bit, linearity of the adjoint)."""
 
+import os
+
 import numpy as np
 import pytest
 
@@ -208,6 +210,102 @@ def test_full_size_invariants(device, qh):
     assert torch.equal(stored.vjp(x, r1), g1)
     del stored
     torch.cuda.synchronize()
+
+
+def overlapping_beads(ph):
+    """Three overlapping beads, radii 2-4 um, Delta n up to 0.048 above the medium."""
+    return [ph.Sphere(0.0, 0.0, 0.0, 4.0, 1.385), ph.Sphere(2.5, 1.5, -1.0, 3.0, 1.372),
+            ph.Sphere(-2.0, -2.0, 2.0, 2.0, 1.360)]
+
+
+def test_gradient_at_full_size(device, qh, oracle):
+    """The BPM adjoint workload at its own size: 256^2 x 256 at 0.2 um, 49 illuminations at NA
+    0.9, three overlapping beads, checkpoints every 16 slices. The data-term gradient at 0.7
+    times the true contrast, on a subset of illuminations [first, last) (the device plan holds
+    all 49), against the oracle's float gradient on two subsets, and against central
+    differences (step 1e-4) of the FP64 oracle's loss on illumination 0 at 64 seeded voxels
+    (the reference's pattern, tests/test_cuda.cpp:149-160 and tests/test_bpm.cpp:108-126)."""
+    torch = pytest.importorskip("torch")
+    from concurrent.futures import ThreadPoolExecutor
+
+    from qoraholo_amd import phantom as ph
+
+    n = nz = 256
+    g = qh.Geometry(*GEOM)
+    il = qh.circular_scan(49, 0.9, g)
+    x_true = (ph.render_phantom(overlapping_beads(ph), g, nz, n) - np.float32(GEOM[1]))
+    x_true = x_true.astype(np.float32)
+    print("bpm gradient at full size 256^3 x 49", flush=True)
+    plan = qh.BpmPlan(nz, n, il, g)
+    data = plan.forward(torch.from_numpy(x_true).cuda()).cpu().numpy()
+    x = (0.7 * x_true).astype(np.float32)
+    xt = torch.from_numpy(x).cuda()
+    dt = torch.from_numpy(data).cuda()
+    for first, last in ((0, 2), (30, 33)):
+        grad, _ = plan.gradient(xt, dt, first, last)
+        want, _ = oracle.bpm_gradient(x, data, il, GEOM, first=first, last=last)
+        err = rel_l2(grad.cpu().numpy(), want)
+        print(f"  illuminations [{first}, {last}): rel L2 vs oracle {err:.2e}", flush=True)
+        assert err < 2e-5
+    grad, _ = plan.gradient(xt, dt, 0, 1)
+    grad = grad.cpu().numpy().reshape(-1)
+    del plan
+    torch.cuda.synchronize()
+
+    x64 = x.astype(np.float64)
+    d0 = data[:1].astype(np.complex128)
+    il0 = il[:1]
+
+    def loss(i, sign):
+        oracle.set_threads(1)
+        xp = x64.copy().reshape(-1)
+        xp[i] += sign * 1e-4
+        y = oracle.bpm_forward(xp.reshape(x.shape), il0, GEOM, dtype=np.float64)
+        return 0.5 * float(np.sum(np.abs(y - d0) ** 2))
+
+    voxels = np.random.default_rng(11).choice(x.size, 64, replace=False)
+    jobs = [(int(i), s) for i in voxels for s in (1.0, -1.0)]
+    with ThreadPoolExecutor(max_workers=min(16, len(os.sched_getaffinity(0)))) as ex:
+        values = list(ex.map(lambda job: loss(*job), jobs))
+    scale = np.abs(grad).max()
+    for k, i in enumerate(voxels):
+        fd = (values[2 * k] - values[2 * k + 1]) / 2e-4
+        assert grad[i] == pytest.approx(fd, rel=2e-3, abs=1e-4 * scale), (int(i), grad[i], fd)
+
+
+def test_served_request_at_384(device, qh, oracle):
+    """The served BPM inversion at its larger size: 384^2 x 384 with 81 illuminations through
+    the Server (Rytov, GP 20 start, 2 FISTA iterations with the TV prox of weight 1e-2). The
+    served result equals the per-call chain bit for bit; the loss falls and n >= n_m; and the
+    device gradient at the start, on 2 of the 81 illuminations, matches the oracle's."""
+    torch = pytest.importorskip("torch")
+    from qoraholo_amd import workloads
+
+    req = workloads.bpm_request(384, 81, seed=0)
+    g, il, f = req.geometry, req.illuminations, req.fields
+    nm = np.float32(GEOM[1])
+    print("bpm served request 384^3 x 81", flush=True)
+    server = qh.Server(max_slots=1)
+    gid = server.add_bpm(384, il, g, start_iterations=20, iterations=2, tv_weight=1e-2,
+                         tv_inner=10)
+    served, served_loss, _ = server.wait(server.submit_bpm(gid, f))
+    del server
+    start = qh.reconstruct(qh.rytov(f), il, g, iterations=20)
+    ri, loss = qh.bpm_reconstruct(f, il, g, start, iterations=2, tv_weight=1e-2, tv_inner=10,
+                                  return_loss=True)
+    assert np.array_equal(served, ri)
+    assert np.array_equal(served_loss, loss)
+    assert loss[-1] < loss[0]
+    assert float(ri.min()) >= nm
+
+    x = (start - nm).astype(np.float32)
+    plan = qh.BpmPlan(384, 384, il, g)
+    grad, _ = plan.gradient(torch.from_numpy(x).cuda(), torch.from_numpy(f).cuda(), 40, 42)
+    del plan
+    want, _ = oracle.bpm_gradient(x, f, il, GEOM, first=40, last=42)
+    err = rel_l2(grad.cpu().numpy(), want)
+    print(f"  gradient on illuminations [40, 42): rel L2 vs oracle {err:.2e}", flush=True)
+    assert err < 2e-5
 
 
 def test_inversion_of_exact_bead_on_device(device, qh, oracle):

@@ -327,6 +327,35 @@ def test_pipeline_graph_replays_equal_kernel_launches(device, qh, solver):
         assert 0 < t["fgp_ms"] < t["solve"] and t["fgp_steps"] == 4 * 10 and t["fields"] > 0
 
 
+@pytest.mark.parametrize("graph", [True, False])
+def test_pipeline_orders_requests_across_streams(device, qh, graph):
+    """Requests share the pipeline's device buffers: run_device on a side stream followed at
+    once (no synchronisation) by run on the pipeline's own streams, then run_device on another
+    stream, each give the result of a request run alone."""
+    torch = pytest.importorskip("torch")
+    d = PipelineData(qh, iterations=4, solver="tv")
+    p = qh.Pipeline(d.count, d.size, illuminations=d.illum, geometry=d.geom, **d.kw)
+    p.set_background(d.bg)
+    p.use_graph(graph)
+    print(f"pipeline stream order graph={graph}", flush=True)
+    alone = p.run(d.holo)
+    holo_t = torch.from_numpy(d.holo).to("cuda:0")
+    out1 = torch.empty(p.output_shape, dtype=torch.float32, device="cuda:0")
+    out2 = torch.empty_like(out1)
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    s1.wait_stream(torch.cuda.current_stream())
+    s2.wait_stream(torch.cuda.current_stream())
+    for _ in range(2):
+        p.run_device(holo_t, out1, stream=s1)
+        host = p.run(d.holo)
+        p.run_device(holo_t, out2, stream=s2)
+        p.run_device(holo_t, out1, stream=s1)
+        torch.cuda.synchronize()
+        assert np.array_equal(host, alone)
+        assert np.array_equal(out1.cpu().numpy(), alone)
+        assert np.array_equal(out2.cpu().numpy(), alone)
+
+
 def test_pipeline_needs_background(device, qh):
     d = PipelineData(qh, angles=3)
     p = qh.Pipeline(d.count, d.size, illuminations=d.illum, geometry=d.geom, **d.kw)
