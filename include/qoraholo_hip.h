@@ -388,6 +388,20 @@ int qh_hip_irfft3(const qh_c64* in, int nz, int ny, int nx, float scale, float* 
 int qh_hip_tv_prox(const float* b, int nz, int ny, int nx, float lambda, int inner, int fused,
                    float* x);
 
+/* ---- allocation-free variant (no counterpart in the reference) ----
+ * qh_hip_tv_prox for callers that cannot allocate device memory (code under hipGraph capture,
+ * tight serving loops, harnesses that forbid hipMalloc): the same result bits, with b and x
+ * device pointers of nz*ny*nx floats and the work queued on `stream` (NULL: the null stream);
+ * the call returns without waiting for it. The FGP dual lives in a fixed device array of the
+ * library, 96 MiB on each device it runs on, held until the process ends; volumes of more than
+ * QH_TV_NOALLOC_MAX_VOXELS voxels are refused with QH_ERR_INVALID_ARGUMENT. The call makes no
+ * hipMalloc or hipFree. Every call on one device shares that array (each call clears the dual
+ * first): calls that overlap in time on one device, from other streams or host threads, race
+ * and must be ordered by the caller. */
+#define QH_TV_NOALLOC_MAX_VOXELS 2097152
+int qh_hip_tv_prox_dev_noalloc(const float* b, int nz, int ny, int nx, float lambda, int inner,
+                               int fused, float* x, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
