@@ -346,7 +346,7 @@ def bpm_leg(qh, torch, cpu=True, repeats=5):
     return out
 
 
-def served_mixed(qh, rates=(0.25, 0.5, 1.0, 2.0), per_rate=40, seed=0, max_slots=8):
+def served_mixed(qh, rates=(0.25, 0.5, 1.0, 2.0), per_rate=100, seed=0, max_slots=8):
     """Config 3: a served stream of mixed shapes and solvers at Poisson rates set against the
     plain path's measured capacity. Shapes uniform over 25 x 256^2 -> 128^3, 49 x 512^2 -> 256^3
     and 81 x 768^2 -> 384^3 (random cells of 3-8 nested or disjoint spheres), solvers uniform
@@ -431,7 +431,7 @@ def served_mixed(qh, rates=(0.25, 0.5, 1.0, 2.0), per_rate=40, seed=0, max_slots
 
 
 def served_bpm(qh, sizes=((192, 49), (384, 81)), iterations=40, seed=0, repeats=None,
-               max_slots=4, stream=True):
+               max_slots=4, stream=True, poisson_load=None):
     """Config 5: BPM inversion requests from host arrays. Each request is the reference's recipe
     (examples/bpm_figure.py:74-88): Rytov of the field ratios, 20 GP iterations for the start,
     then 40 full-batch FISTA iterations of the BPM with the TV prox (weight 1e-2, 10 inner
@@ -442,7 +442,9 @@ def served_bpm(qh, sizes=((192, 49), (384, 81)), iterations=40, seed=0, repeats=
     the same requests through the Server, its BPM geometries resident and admitted by their
     measured HBM footprint: a burst of `repeats[n]` requests of side n (4 of 192, 2 of 384),
     interleaved, with inversions/s, p50/p99 latency and each result compared with the per-call
-    one."""
+    one. With `poisson_load`, the requests arrive instead by a Poisson process at that fraction
+    of the per-call capacity (1 / mean per-call request time), so that the latency percentiles
+    are those of a loaded server rather than of a queue drained from a burst."""
     from qoraholo_amd import workloads
 
     out = []
@@ -512,16 +514,31 @@ def served_bpm(qh, sizes=((192, 49), (384, 81)), iterations=40, seed=0, repeats=
     for i in range(max(counts)):
         order += [k for k in range(len(sizes)) if i < counts[k]]
     log(f"served bpm stream: {len(order)} requests, sizes {[sizes[k][0] for k in order]}")
+    plain_s = sum(out[k]["request_ms"] for k in order) / 1000.0
     t0 = time.perf_counter()
-    ids = [(k, server.submit_bpm(gids[k], reqs[k][0].fields)) for k in order]
+    if poisson_load:
+        rate = poisson_load * len(order) / plain_s
+        gaps = np.random.default_rng(seed).exponential(1.0 / rate, len(order))
+        ids = []
+        tick = time.perf_counter()
+        for k, gap in zip(order, gaps):
+            tick += gap
+            delay = tick - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            ids.append((k, server.submit_bpm(gids[k], reqs[k][0].fields)))
+    else:
+        ids = [(k, server.submit_bpm(gids[k], reqs[k][0].fields)) for k in order]
     same = True
     for k, rid in ids:
         ri, loss, _ = server.wait(rid)
         same = same and bool(np.array_equal(ri, reqs[k][1])) and bool(np.array_equal(loss, reqs[k][2]))
     wall = time.perf_counter() - t0
     st = server.stats()
-    plain_s = sum(out[k]["request_ms"] for k in order) / 1000.0
-    stream = {"requests": len(order), "sizes": [sizes[k][0] for k in order],
+    stream = {"requests": len(order), "sizes": sorted(set(sizes[k][0] for k in order)),
+              "arrivals": (f"Poisson at {poisson_load} x the per-call capacity "
+                           f"({poisson_load * len(order) / plain_s:.3f}/s)" if poisson_load
+                           else "burst"),
               "inversions_per_s": round(st["recon_per_s"], 4),
               "wall_s": round(wall, 2),
               "p50_ms": round(st["p50_ms"], 1), "p99_ms": round(st["p99_ms"], 1),
@@ -537,7 +554,8 @@ def pct(v, q):
     return round(float(np.percentile(v, q)), 3)
 
 
-def serving_comparison(qh, w, plain_requests=3, serial_requests=5, served_requests=16, seed=0):
+def serving_comparison(qh, w, plain_requests=100, serial_requests=100, served_requests=100,
+                       seed=0):
     """Requests from host memory to host memory on the same workload:
     - plain: the per-call chain a user of the reference's API runs (extract_fields for sample
       and background, field_ratio on the host, rytov, reconstruct), serially;
@@ -649,6 +667,11 @@ def main() -> int:
                     help="with --served-bpm: only the request of this size")
     ap.add_argument("--no-served-stream", action="store_true",
                     help="with --served-bpm: only the per-call requests, not the served stream")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the mixed served stream (configuration 3) in the default run")
+    ap.add_argument("--no-served-bpm", action="store_true",
+                    help="skip the served 192^3 x 49 BPM inversions (configuration 5) in the "
+                         "default run")
     ap.add_argument("--fgp-traffic", action="store_true",
                     help="run only 5 FGP steps at 256^3 and a 256 MiB device copy, for the "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE runs that calibrate HBM traffic")
@@ -820,8 +843,18 @@ def main() -> int:
         result["latency_ms"]["served"] = {"p50": serving["served_poisson"]["p50_ms"],
                                           "p99": serving["served_poisson"]["p99_ms"]}
 
+    if not args.no_mixed:
+        # Configuration 3 at 1x and 2x the plain path's capacity, 100 Poisson arrivals each.
+        result["served_mixed"] = served_mixed(qh, rates=(1.0, 2.0), per_rate=100)
+
     if not args.no_bpm:
         result["bpm"] = bpm_leg(qh, torch, cpu=not args.no_cpu_baseline)
+
+    if not args.no_served_bpm:
+        # Configuration 5's 192^2 x 192 / 49-illumination requests: 100 Poisson arrivals at 0.8x
+        # the per-call capacity (the 384^3 / 81 requests take 9 s each: --served-bpm).
+        result["served_bpm"] = served_bpm(qh, sizes=((192, 49),), repeats={192: 100},
+                                          poisson_load=0.8)
 
     if not args.no_cpu_baseline:
         result["cpu_baseline"], want = cpu_baseline(w)

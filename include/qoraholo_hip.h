@@ -263,6 +263,7 @@ typedef struct {
   int slots;           /* slots resident on the device (idle ones keep their geometry's state) */
   int busy_slots;      /* slots running a request now */
   int peak_busy_slots; /* most slots running requests at once since the last reset */
+  int graph_slots;     /* resident slots whose requests replay one captured hipGraph */
 } qh_server_stats;
 
 /* budget_bytes 0: 90% of the free device memory. max_slots: requests in flight at once; idle
@@ -352,6 +353,39 @@ int qh_hip_bpm_reconstruct(const qh_c64* fields, int count, int n, const qh_illu
                            const qh_geometry* geometry, float* x, int nz,
                            const qh_bpm_options* options, const qh_bpm_solve_options* solve,
                            double* loss);
+/* bpm_jvp (src/cpu/bpm.cpp:141-176, CPU-only in the reference, src/backends.hpp:100-101): the
+ * tangent fields J h (count, n, n) of the model linearised at x, for a direction h (nz, n, n). */
+int qh_hip_bpm_jvp(const float* x, const float* h, int nz, int n, const qh_illumination* illum,
+                   int count, const qh_geometry* geometry, const qh_bpm_options* options,
+                   qh_c64* out);
+
+/* The tables as src/backends.hpp:61-70 (BpmTables) holds them, borrowed for the call: step and
+ * last (count, n, n) complex, q (count, n, n), lateral bins in FFT order, the screen phase
+ * k0 pixel and the Lipschitz constant (what qh_bpm_tables returns). */
+typedef struct {
+  int n;
+  int nz;
+  int count;
+  float phase;
+  const qh_c64* step;
+  const qh_c64* last;
+  const float* q;
+  float lipschitz;
+} qh_bpm_table_view;
+/* The same four calls on tables, with the arguments of cuda::bpm_forward(x, BpmTables),
+ * cuda::bpm_vjp(x, r, BpmTables) and cuda::bpm_reconstruct(fields, x, BpmTables, options,
+ * chunk) (src/backends.hpp:119-124) and cpu::bpm_jvp(x, h, BpmTables) (:100): volumes are
+ * (tables->nz, n, n), fields (tables->count, n, n). options->obliquity is not read (the tables
+ * carry it); chunk and checkpoint are the device layout as above. */
+int qh_hip_bpm_forward_tables(const float* x, const qh_bpm_table_view* tables,
+                              const qh_bpm_options* options, qh_c64* out);
+int qh_hip_bpm_vjp_tables(const float* x, const qh_c64* r, const qh_bpm_table_view* tables,
+                          const qh_bpm_options* options, float* grad);
+int qh_hip_bpm_jvp_tables(const float* x, const float* h, const qh_bpm_table_view* tables,
+                          const qh_bpm_options* options, qh_c64* out);
+int qh_hip_bpm_reconstruct_tables(const qh_c64* fields, float* x,
+                                  const qh_bpm_table_view* tables, const qh_bpm_options* options,
+                                  const qh_bpm_solve_options* solve, double* loss);
 
 /* Device-resident model for callers that keep x, fields and gradients in device memory (e.g.
  * a torch.autograd.Function): tables and work buffers stay on the device between calls. The
@@ -368,6 +402,8 @@ int qh_hip_bpm_plan_info(const qh_bpm_plan* plan, float* phase, float* lipschitz
                          size_t* device_bytes);
 int qh_hip_bpm_forward_dev(qh_bpm_plan* plan, const float* x, qh_c64* out, void* stream);
 int qh_hip_bpm_vjp_dev(qh_bpm_plan* plan, const float* x, const qh_c64* r, float* grad,
+                       void* stream);
+int qh_hip_bpm_jvp_dev(qh_bpm_plan* plan, const float* x, const float* h, qh_c64* out,
                        void* stream);
 /* gradient() of src/cpu/bpm.cpp:100-129: grad = Re(J^H (bpm(x) - fields)) / (last - first)
  * over illuminations [first, last); *loss = 0.5 / count times the sum of squared residuals.

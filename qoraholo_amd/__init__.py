@@ -23,11 +23,12 @@ import numpy as np
 
 from . import _native
 from ._native import LogicError, OutOfDeviceMemory
-from .bpm import BpmPlan, bpm, bpm_forward, bpm_reconstruct, bpm_tables, bpm_vjp
+from .bpm import BpmPlan, BpmTables, bpm, bpm_forward, bpm_jvp, bpm_reconstruct, bpm_tables, bpm_vjp
 from .phantom import Ellipsoid, Sphere, forward_fields, render_phantom, synthesize_holograms
 
 __all__ = [
     "BpmPlan",
+    "BpmTables",
     "Ellipsoid",
     "Geometry",
     "LogicError",
@@ -38,6 +39,7 @@ __all__ = [
     "Sphere",
     "bpm",
     "bpm_forward",
+    "bpm_jvp",
     "bpm_reconstruct",
     "bpm_tables",
     "bpm_vjp",

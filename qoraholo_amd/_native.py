@@ -62,11 +62,16 @@ class ServerStats_t(ctypes.Structure):
                 ("failed", ctypes.c_int64), ("queued", ctypes.c_int64), ("p50_ms", c_double),
                 ("p99_ms", c_double), ("mean_ms", c_double), ("recon_per_s", c_double),
                 ("device_bytes", c_size_t), ("budget_bytes", c_size_t), ("slots", c_int),
-                ("busy_slots", c_int), ("peak_busy_slots", c_int)]
+                ("busy_slots", c_int), ("peak_busy_slots", c_int), ("graph_slots", c_int)]
 
 
 class BpmOptions_t(ctypes.Structure):
     _fields_ = [("obliquity", c_int), ("chunk", c_int), ("checkpoint", c_int)]
+
+
+class BpmTableView_t(ctypes.Structure):
+    _fields_ = [("n", c_int), ("nz", c_int), ("count", c_int), ("phase", c_float),
+                ("step", c_void_p), ("last", c_void_p), ("q", c_void_p), ("lipschitz", c_float)]
 
 
 class BpmSolveOptions_t(ctypes.Structure):
@@ -142,6 +147,17 @@ _SIGNATURES = {
     "qh_hip_bpm_reconstruct": (c_int, [c_void_p, c_int, c_int, c_void_p, POINTER(Geometry_t),
                                        c_void_p, c_int, POINTER(BpmOptions_t),
                                        POINTER(BpmSolveOptions_t), c_void_p]),
+    "qh_hip_bpm_jvp": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_int,
+                               POINTER(Geometry_t), POINTER(BpmOptions_t), c_void_p]),
+    "qh_hip_bpm_forward_tables": (c_int, [c_void_p, POINTER(BpmTableView_t), POINTER(BpmOptions_t),
+                                          c_void_p]),
+    "qh_hip_bpm_vjp_tables": (c_int, [c_void_p, c_void_p, POINTER(BpmTableView_t),
+                                      POINTER(BpmOptions_t), c_void_p]),
+    "qh_hip_bpm_jvp_tables": (c_int, [c_void_p, c_void_p, POINTER(BpmTableView_t),
+                                      POINTER(BpmOptions_t), c_void_p]),
+    "qh_hip_bpm_reconstruct_tables": (c_int, [c_void_p, c_void_p, POINTER(BpmTableView_t),
+                                              POINTER(BpmOptions_t), POINTER(BpmSolveOptions_t),
+                                              c_void_p]),
     "qh_hip_bpm_plan_create": (c_int, [c_int, c_int, c_void_p, c_int, POINTER(Geometry_t),
                                        POINTER(BpmOptions_t), POINTER(c_void_p)]),
     "qh_hip_bpm_plan_destroy": (None, [c_void_p]),
@@ -149,6 +165,7 @@ _SIGNATURES = {
                                      POINTER(c_size_t)]),
     "qh_hip_bpm_forward_dev": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "qh_hip_bpm_vjp_dev": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "qh_hip_bpm_jvp_dev": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "qh_hip_bpm_gradient_dev": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p,
                                         POINTER(c_double), c_void_p]),
     "qh_hip_fft_c2c": (c_int, [c_void_p, c_int, c_int, c_int, c_uint, c_int, c_float]),

@@ -108,6 +108,105 @@ def bpm_vjp(x, r, illuminations, geometry, obliquity: bool = True, chunk: int = 
     return grad
 
 
+def bpm_jvp(x, h, illuminations, geometry, obliquity: bool = True, chunk: int = 0,
+            checkpoint: int = DEFAULT_CHECKPOINT):
+    """J h: the model linearised at the contrast x, applied to a direction h (nz, n, n); the
+    tangent fields (count, n, n) (the reference's cpu::bpm_jvp, src/cpu/bpm.cpp:141-176)."""
+    x = _volume(x, "bpm_jvp")
+    h = _volume(h, "bpm_jvp")
+    if h.shape != x.shape:
+        raise ValueError("bpm_jvp: h must have the shape of x")
+    il = _illum(illuminations)
+    nz, n, _ = x.shape
+    out = np.empty((il.shape[0], n, n), np.complex64)
+    _native.check(_native.lib().qh_hip_bpm_jvp(
+        _native.ptr(x), _native.ptr(h), nz, n, _native.ptr(il), il.shape[0],
+        ctypes.byref(geometry._c()), ctypes.byref(_opts(obliquity, chunk, checkpoint)),
+        _native.ptr(out)))
+    return out
+
+
+class BpmTables:
+    """The tables of one volume shape and illumination set (the reference's BpmTables,
+    src/backends.hpp:61-70): step, last (count, n, n) complex64, q (count, n, n) float32, the
+    screen phase and the Lipschitz constant. The ``*_tables`` calls run the model on them, as
+    the reference's backends take them (src/backends.hpp:119-124)."""
+
+    def __init__(self, step, last, q, phase: float, lipschitz: float, nz: int):
+        self.step = np.ascontiguousarray(step, dtype=np.complex64)
+        self.last = np.ascontiguousarray(last, dtype=np.complex64)
+        self.q = np.ascontiguousarray(q, dtype=np.float32)
+        if self.step.ndim != 3 or self.step.shape != self.last.shape or self.step.shape != self.q.shape:
+            raise ValueError("BpmTables: step, last and q must be (count, n, n)")
+        self.phase = float(phase)
+        self.lipschitz = float(lipschitz)
+        self.nz = int(nz)
+        self.count, self.n, _ = self.step.shape
+
+    @classmethod
+    def build(cls, n: int, nz: int, illuminations, geometry, obliquity: bool = True):
+        step, last, q, phase, lip = bpm_tables(n, nz, illuminations, geometry, obliquity)
+        return cls(step, last, q, phase, lip, nz)
+
+    def _c(self):
+        return _native.BpmTableView_t(self.n, self.nz, self.count, self.phase,
+                                      self.step.ctypes.data, self.last.ctypes.data,
+                                      self.q.ctypes.data, self.lipschitz)
+
+    def _x(self, x, who):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        if x.shape != (self.nz, self.n, self.n):
+            raise ValueError(f"{who}: volume must be (nz, n, n) of the tables")
+        return x
+
+    def _fields(self, f, who):
+        f = np.ascontiguousarray(f, dtype=np.complex64)
+        if f.shape != (self.count, self.n, self.n):
+            raise ValueError(f"{who}: fields must be (count, n, n) of the tables")
+        return f
+
+    def forward(self, x, chunk: int = 0, checkpoint: int = DEFAULT_CHECKPOINT):
+        x = self._x(x, "bpm_forward")
+        out = np.empty((self.count, self.n, self.n), np.complex64)
+        _native.check(_native.lib().qh_hip_bpm_forward_tables(
+            _native.ptr(x), ctypes.byref(self._c()), ctypes.byref(_opts(True, chunk, checkpoint)),
+            _native.ptr(out)))
+        return out
+
+    def vjp(self, x, r, chunk: int = 0, checkpoint: int = DEFAULT_CHECKPOINT):
+        x = self._x(x, "bpm_vjp")
+        r = self._fields(r, "bpm_vjp")
+        grad = np.empty_like(x)
+        _native.check(_native.lib().qh_hip_bpm_vjp_tables(
+            _native.ptr(x), _native.ptr(r), ctypes.byref(self._c()),
+            ctypes.byref(_opts(True, chunk, checkpoint)), _native.ptr(grad)))
+        return grad
+
+    def jvp(self, x, h, chunk: int = 0, checkpoint: int = DEFAULT_CHECKPOINT):
+        x = self._x(x, "bpm_jvp")
+        h = self._x(h, "bpm_jvp")
+        out = np.empty((self.count, self.n, self.n), np.complex64)
+        _native.check(_native.lib().qh_hip_bpm_jvp_tables(
+            _native.ptr(x), _native.ptr(h), ctypes.byref(self._c()),
+            ctypes.byref(_opts(True, chunk, checkpoint)), _native.ptr(out)))
+        return out
+
+    def reconstruct(self, fields, x, iterations: int = 20, batch: int = 0, step: float = 1.0,
+                    tv_weight: float = 0.0, tv_inner: int = 10, chunk: int = 0,
+                    checkpoint: int = DEFAULT_CHECKPOINT):
+        """(contrast, loss) from the starting contrast x, as cuda::bpm_reconstruct."""
+        f = self._fields(fields, "bpm_reconstruct")
+        x = self._x(x, "bpm_reconstruct").copy()
+        solve = _native.BpmSolveOptions_t(int(iterations), int(batch), float(step),
+                                          float(tv_weight), int(tv_inner))
+        loss = np.zeros(max(int(iterations), 0), np.float64)
+        _native.check(_native.lib().qh_hip_bpm_reconstruct_tables(
+            _native.ptr(f), _native.ptr(x), ctypes.byref(self._c()),
+            ctypes.byref(_opts(True, chunk, checkpoint)), ctypes.byref(solve),
+            _native.ptr(loss) if loss.size else None))
+        return x, loss
+
+
 def bpm_reconstruct(fields, illuminations, geometry, initial, iterations: int = 20, batch: int = 0,
                     step: float = 1.0, tv_weight: float = 0.0, tv_inner: int = 10,
                     obliquity: bool = True, backend: str = "hip", return_loss: bool = False,
@@ -233,6 +332,19 @@ class BpmPlan:
             out = torch.empty((self.nz, self.n, self.n), dtype=torch.float32, device=x.device)
         self._check_x(out)
         _native.check(_native.lib().qh_hip_bpm_vjp_dev(self._h, _addr(x), _addr(r), _addr(out),
+                                                       _stream(stream)))
+        return out
+
+    def jvp(self, x, h, out=None, stream=None):
+        """J h at the contrast x: tangent fields (count, n, n)."""
+        import torch
+
+        self._check_x(x)
+        self._check_x(h)
+        if out is None:
+            out = torch.empty((self.count, self.n, self.n), dtype=torch.complex64, device=x.device)
+        self._check_fields(out, "out")
+        _native.check(_native.lib().qh_hip_bpm_jvp_dev(self._h, _addr(x), _addr(h), _addr(out),
                                                        _stream(stream)))
         return out
 

@@ -98,6 +98,9 @@ struct BpmModel::Args {
   float inv;             // 1 / n^2
   float phase;
   int first;             // adjoint: the first slice processed (no state yet)
+  int s_zero;            // columns with s_in == null: the state in front of slice 0 is zero
+                         // (the tangent), not the unit envelope
+  const float* h;        // tangent direction of the slice, (n, n)
 };
 
 namespace {
@@ -133,6 +136,33 @@ __global__ void __launch_bounds__(E::kBlock) k_bpm_rows_fwd(Args a) {
   e.store2([&](long l, int m, float2 v) { a.out[l * n + m] = v; });
 }
 
+// Tangent rows (bpm_jvp, src/cpu/bpm.cpp:141-176): inverse x transform of the tangent dT,
+// dw = dT / n^2, dv = t (dw + i phase h w) with the envelope w the primal rows stored for this
+// slice, forward x transform. in == null: dw = 0 (the first slice).
+template <class E>
+__global__ void __launch_bounds__(E::kBlock) k_bpm_rows_jvp(Args a) {
+  E e(a.g);
+  const int n = a.g.n;
+  // The complex operations of the reference, in its order: (0, phase h) w, + dw, t times that.
+  const auto tangent = [&](long l, int k, float2 dw) {
+    const float2 w = a.w[l * n + k];
+    const float p = a.phase * a.h[(l % n) * n + k];
+    const float2 d{dw.x + (0.0f * w.x - p * w.y), dw.y + (0.0f * w.y + p * w.x)};
+    return cmul(a.screen[(l % n) * n + k], d);
+  };
+  if (a.in == nullptr) {
+    e.load([&](long l, int i) { return tangent(l, i, float2{0.0f, 0.0f}); });
+    e.first(false);
+    e.store1([&](long l, int k, float2 v) { a.out[l * n + k] = v; });
+    return;
+  }
+  e.load([&](long l, int i) { return a.in[l * n + i]; });
+  e.first(true);
+  e.mid([&](long l, int k, float2& v) { v = tangent(l, k, cscale(v, a.inv)); });
+  e.second(false);
+  e.store2([&](long l, int m, float2 v) { a.out[l * n + m] = v; });
+}
+
 __device__ __forceinline__ long col_at(long l, int i, int n, long plane) {
   return (l / n) * plane + static_cast<long>(i) * n + (l % n);
 }
@@ -151,6 +181,8 @@ __global__ void __launch_bounds__(E::kBlock) k_bpm_cols_fwd(Args a) {
     float2 s;
     if (a.s_in) {
       s = a.s_in[x];
+    } else if (a.s_zero) {
+      s = float2{0.0f, 0.0f};
     } else {
       s = (ky == 0 && l % n == 0) ? float2{static_cast<float>(plane), 0.0f} : float2{0.0f, 0.0f};
     }
@@ -313,7 +345,7 @@ __global__ void __launch_bounds__(256) k_bpm_clamp(float* x, long total) {
   x[i] = v < 0.0f ? 0.0f : v;
 }
 
-enum class Kind { rows_fwd, rows_out, rows_adj, cols_fwd, cols_adj };
+enum class Kind { rows_fwd, rows_out, rows_adj, rows_jvp, cols_fwd, cols_adj };
 
 template <bool ROWS, class E>
 void launch_with(Kind k, Args a, unsigned grid, std::size_t lds, hipStream_t s) {
@@ -321,6 +353,7 @@ void launch_with(Kind k, Args a, unsigned grid, std::size_t lds, hipStream_t s) 
     switch (k) {
       case Kind::rows_fwd: hipLaunchKernelGGL(k_bpm_rows_fwd<E>, dim3(grid), dim3(E::kBlock), lds, s, a); break;
       case Kind::rows_out: hipLaunchKernelGGL(k_bpm_rows_out<E>, dim3(grid), dim3(E::kBlock), lds, s, a); break;
+      case Kind::rows_jvp: hipLaunchKernelGGL(k_bpm_rows_jvp<E>, dim3(grid), dim3(E::kBlock), lds, s, a); break;
       default: hipLaunchKernelGGL(k_bpm_rows_adj<E>, dim3(grid), dim3(E::kBlock), lds, s, a); break;
     }
   } else {
@@ -543,6 +576,50 @@ void BpmModel::vjp(const float* x, const float2* r, float* grad, hipStream_t s) 
     forward_pass(a0, b, true, s);
     fft_lines(r + a0 * plane_, a_.get(), n_, lines_along(2, 1, b * n_, n_, n_), false, 1.0f, 0, s);
     adjoint(a0, b, grad, s);
+  }
+}
+
+// Primal and tangent march slice by slice: the primal rows store the slice's envelope in w_,
+// the tangent rows read it; T, S in f_, s_ and dT, dS in a_, g_ (the adjoint's buffers).
+void BpmModel::jvp(const float* x, const float* h, float2* y, hipStream_t s) {
+  ensure_adjoint();
+  set_screen(x, s);
+  for (int a0 = 0; a0 < count_; a0 += chunk_) {
+    const int b = std::min(chunk_, count_ - a0);
+    for (int k = 0; k < nz_; ++k) {
+      Args r = base(b);
+      r.in = k == 0 ? nullptr : f_.get();
+      r.out = f_.get();
+      r.screen = screen_.get() + k * plane_;
+      r.w = w_.get();
+      launch<true>(Kind::rows_fwd, r, s);
+      Args t = base(b);
+      t.in = k == 0 ? nullptr : a_.get();
+      t.out = a_.get();
+      t.screen = screen_.get() + k * plane_;
+      t.w = w_.get();
+      t.h = h + k * plane_;
+      launch<true>(Kind::rows_jvp, t, s);
+      Args c = base(b);
+      c.in = f_.get();
+      c.out = f_.get();
+      c.s_in = k == 0 ? nullptr : s_.get();
+      c.s_out = s_.get();
+      c.c = (k == nz_ - 1 ? last_.get() : step_.get()) + a0 * plane_;
+      c.q = q_.get() + a0 * plane_;
+      launch<false>(Kind::cols_fwd, c, s);
+      Args d = c;
+      d.in = a_.get();
+      d.out = a_.get();
+      d.s_in = k == 0 ? nullptr : g_.get();
+      d.s_out = g_.get();
+      d.s_zero = 1;
+      launch<false>(Kind::cols_fwd, d, s);
+    }
+    Args r = base(b);
+    r.in = a_.get();
+    r.out = y + a0 * plane_;
+    launch<true>(Kind::rows_out, r, s);
   }
 }
 

@@ -58,6 +58,9 @@ class BpmModel {
   void forward(const float* x, float2* y, hipStream_t s);
   // grad (nz, n, n) = Re(J^H r) at x, summed over all illuminations in order.
   void vjp(const float* x, const float2* r, float* grad, hipStream_t s);
+  // Tangent fields y (count, n, n) = J h at x: the linearised model of bpm_jvp
+  // (src/cpu/bpm.cpp:141-176), marched beside the forward model slice by slice.
+  void jvp(const float* x, const float* h, float2* y, hipStream_t s);
   // grad = sum over a in [first, last) of Re(J_a^H (bpm_a(x) - fields_a)), unscaled;
   // loss_lines[a n + y] = sum over x of |bpm_a(x) - fields_a|^2 in double, per image row.
   void gradient(const float* x, const float2* fields, int first, int last, float* grad,

@@ -166,6 +166,31 @@ BpmTables checked_bpm_tables(int nz, int n, int count, const qh_illumination* il
   return bpm_tables(n, nz, illum, count, *g, o->obliquity != 0);
 }
 
+// BpmTables from the caller's arrays (the layout of src/backends.hpp:61-70), with the shape
+// checks of checked_bpm_tables.
+BpmTables tables_from_view(const qh_bpm_table_view* v, const qh_bpm_options* o, const char* who) {
+  const std::string w(who);
+  require(v != nullptr && o != nullptr, w + ": missing tables or options");
+  require(v->n >= 4 && v->n % 2 == 0 && v->nz >= 1, w + ": volume must be (nz, n, n) with an even n");
+  require(v->count >= 1, w + ": need one illumination per field");
+  require(v->step != nullptr && v->last != nullptr && v->q != nullptr, w + ": missing table arrays");
+  require(o->chunk >= 0 && o->checkpoint >= 0, w + ": chunk and checkpoint must not be negative");
+  require_fft(v->n, who);
+  BpmTables t;
+  t.n = v->n;
+  t.nz = v->nz;
+  t.count = v->count;
+  t.phase = v->phase;
+  t.lipschitz = v->lipschitz;
+  const std::size_t total = static_cast<std::size_t>(v->count) * v->n * v->n;
+  const float2* step = reinterpret_cast<const float2*>(v->step);
+  const float2* last = reinterpret_cast<const float2*>(v->last);
+  t.step.assign(step, step + total);
+  t.last.assign(last, last + total);
+  t.q.assign(v->q, v->q + total);
+  return t;
+}
+
 // Complex float division as the reference's host build evaluates std::complex<float> / (GCC's
 // runtime on x86-64, which the reference's CPU loop calls): numerator and denominator in double,
 // the simple formula (double carries the range and precision Smith's scaling would otherwise
@@ -226,6 +251,86 @@ OdtConfig config_from(const qh_odt_options& o, const qh_geometry& g) {
   c.n_medium = g.n_medium;
   return c;
 }
+
+namespace {
+
+void bpm_forward_call(const BpmTables& t, const float* x, const qh_bpm_options* o, qh_c64* out) {
+  require_device();
+  const hipStream_t s = call_stream();
+  BpmModel m(t, o->chunk, o->checkpoint);
+  const std::size_t vol = static_cast<std::size_t>(t.nz) * t.n * t.n;
+  DeviceBuffer<float> dx(vol);
+  DeviceBuffer<float2> dy(static_cast<std::size_t>(t.count) * t.n * t.n);
+  upload(dx.get(), x, dx.bytes(), s);
+  m.forward(dx.get(), dy.get(), s);
+  download(out, dy.get(), dy.bytes(), s);
+  check(hipStreamSynchronize(s), "bpm_forward");
+}
+
+void bpm_vjp_call(const BpmTables& t, const float* x, const qh_c64* r, const qh_bpm_options* o,
+                  float* grad) {
+  require_device();
+  const hipStream_t s = call_stream();
+  BpmModel m(t, o->chunk, o->checkpoint);
+  const std::size_t vol = static_cast<std::size_t>(t.nz) * t.n * t.n;
+  DeviceBuffer<float> dx(vol), dg(vol);
+  DeviceBuffer<float2> dr(static_cast<std::size_t>(t.count) * t.n * t.n);
+  upload(dx.get(), x, dx.bytes(), s);
+  upload(dr.get(), r, dr.bytes(), s);
+  m.vjp(dx.get(), dr.get(), dg.get(), s);
+  download(grad, dg.get(), dg.bytes(), s);
+  check(hipStreamSynchronize(s), "bpm_vjp");
+}
+
+void bpm_jvp_call(const BpmTables& t, const float* x, const float* h, const qh_bpm_options* o,
+                  qh_c64* out) {
+  require_device();
+  const hipStream_t s = call_stream();
+  BpmModel m(t, o->chunk, o->checkpoint);
+  const std::size_t vol = static_cast<std::size_t>(t.nz) * t.n * t.n;
+  DeviceBuffer<float> dx(vol), dh(vol);
+  DeviceBuffer<float2> dy(static_cast<std::size_t>(t.count) * t.n * t.n);
+  upload(dx.get(), x, dx.bytes(), s);
+  upload(dh.get(), h, dh.bytes(), s);
+  m.jvp(dx.get(), dh.get(), dy.get(), s);
+  download(out, dy.get(), dy.bytes(), s);
+  check(hipStreamSynchronize(s), "bpm_jvp");
+}
+
+void check_solve_options(const qh_bpm_solve_options* solve, int n, const char* who) {
+  const std::string w(who);
+  require(solve != nullptr, w + ": missing solver options");
+  require(solve->iterations >= 0 && solve->step > 0 && solve->tv_weight >= 0 &&
+              solve->tv_inner >= 1 && solve->batch >= 0,
+          w + ": invalid solver options");
+  require(solve->tv_weight == 0 || n <= 512, w + ": the TV prox handles n <= 512");
+}
+
+void bpm_reconstruct_call(const BpmTables& t, const qh_c64* fields, float* x,
+                          const qh_bpm_options* o, const qh_bpm_solve_options* solve,
+                          double* loss) {
+  require_device();
+  const hipStream_t s = call_stream();
+  BpmModel m(t, o->chunk, o->checkpoint);
+  const std::size_t vol = static_cast<std::size_t>(t.nz) * t.n * t.n;
+  DeviceBuffer<float> dx(vol);
+  DeviceBuffer<float2> df(static_cast<std::size_t>(t.count) * t.n * t.n);
+  upload(dx.get(), x, dx.bytes(), s);
+  upload(df.get(), fields, df.bytes(), s);
+  BpmSolveConfig c;
+  c.iterations = solve->iterations;
+  c.batch = solve->batch;
+  c.step = solve->step;
+  c.tv_weight = solve->tv_weight;
+  c.tv_inner = solve->tv_inner;
+  std::vector<double> l(std::max(c.iterations, 1));
+  bpm_solve(m, df.get(), dx.get(), c, l.data(), s);
+  download(x, dx.get(), dx.bytes(), s);
+  check(hipStreamSynchronize(s), "bpm_reconstruct");
+  if (c.iterations > 0) std::memcpy(loss, l.data(), c.iterations * sizeof(double));
+}
+
+}  // namespace
 
 }  // namespace qh
 
@@ -781,16 +886,7 @@ int qh_hip_bpm_forward(const float* x, int nz, int n, const qh_illumination* ill
   return guarded([&] {
     require(x && out, "bpm_forward: null buffer");
     const BpmTables t = checked_bpm_tables(nz, n, count, illum, geometry, options, "bpm_forward");
-    require_device();
-    const hipStream_t s = call_stream();
-    BpmModel m(t, options->chunk, options->checkpoint);
-    const std::size_t vol = static_cast<std::size_t>(nz) * n * n;
-    DeviceBuffer<float> dx(vol);
-    DeviceBuffer<float2> dy(static_cast<std::size_t>(count) * n * n);
-    upload(dx.get(), x, dx.bytes(), s);
-    m.forward(dx.get(), dy.get(), s);
-    download(out, dy.get(), dy.bytes(), s);
-    check(hipStreamSynchronize(s), "bpm_forward");
+    bpm_forward_call(t, x, options, out);
   });
 }
 
@@ -800,17 +896,7 @@ int qh_hip_bpm_vjp(const float* x, const qh_c64* r, int nz, int n, const qh_illu
   return guarded([&] {
     require(x && r && grad, "bpm_vjp: null buffer");
     const BpmTables t = checked_bpm_tables(nz, n, count, illum, geometry, options, "bpm_vjp");
-    require_device();
-    const hipStream_t s = call_stream();
-    BpmModel m(t, options->chunk, options->checkpoint);
-    const std::size_t vol = static_cast<std::size_t>(nz) * n * n;
-    DeviceBuffer<float> dx(vol), dg(vol);
-    DeviceBuffer<float2> dr(static_cast<std::size_t>(count) * n * n);
-    upload(dx.get(), x, dx.bytes(), s);
-    upload(dr.get(), r, dr.bytes(), s);
-    m.vjp(dx.get(), dr.get(), dg.get(), s);
-    download(grad, dg.get(), dg.bytes(), s);
-    check(hipStreamSynchronize(s), "bpm_vjp");
+    bpm_vjp_call(t, x, r, options, grad);
   });
 }
 
@@ -820,30 +906,55 @@ int qh_hip_bpm_reconstruct(const qh_c64* fields, int count, int n, const qh_illu
                            double* loss) {
   return guarded([&] {
     require(fields && x && solve && (loss || solve->iterations == 0), "bpm_reconstruct: null buffer");
-    require(solve->iterations >= 0 && solve->step > 0 && solve->tv_weight >= 0 &&
-                solve->tv_inner >= 1 && solve->batch >= 0,
-            "bpm_reconstruct: invalid solver options");
-    require(solve->tv_weight == 0 || n <= 512, "bpm_reconstruct: the TV prox handles n <= 512");
+    check_solve_options(solve, n, "bpm_reconstruct");
     const BpmTables t = checked_bpm_tables(nz, n, count, illum, geometry, options, "bpm_reconstruct");
-    require_device();
-    const hipStream_t s = call_stream();
-    BpmModel m(t, options->chunk, options->checkpoint);
-    const std::size_t vol = static_cast<std::size_t>(nz) * n * n;
-    DeviceBuffer<float> dx(vol);
-    DeviceBuffer<float2> df(static_cast<std::size_t>(count) * n * n);
-    upload(dx.get(), x, dx.bytes(), s);
-    upload(df.get(), fields, df.bytes(), s);
-    BpmSolveConfig c;
-    c.iterations = solve->iterations;
-    c.batch = solve->batch;
-    c.step = solve->step;
-    c.tv_weight = solve->tv_weight;
-    c.tv_inner = solve->tv_inner;
-    std::vector<double> l(std::max(c.iterations, 1));
-    bpm_solve(m, df.get(), dx.get(), c, l.data(), s);
-    download(x, dx.get(), dx.bytes(), s);
-    check(hipStreamSynchronize(s), "bpm_reconstruct");
-    if (c.iterations > 0) std::memcpy(loss, l.data(), c.iterations * sizeof(double));
+    bpm_reconstruct_call(t, fields, x, options, solve, loss);
+  });
+}
+
+int qh_hip_bpm_jvp(const float* x, const float* h, int nz, int n, const qh_illumination* illum,
+                   int count, const qh_geometry* geometry, const qh_bpm_options* options,
+                   qh_c64* out) {
+  return guarded([&] {
+    require(x && h && out, "bpm_jvp: null buffer");
+    const BpmTables t = checked_bpm_tables(nz, n, count, illum, geometry, options, "bpm_jvp");
+    bpm_jvp_call(t, x, h, options, out);
+  });
+}
+
+int qh_hip_bpm_forward_tables(const float* x, const qh_bpm_table_view* tables,
+                              const qh_bpm_options* options, qh_c64* out) {
+  return guarded([&] {
+    require(x && out, "bpm_forward: null buffer");
+    bpm_forward_call(tables_from_view(tables, options, "bpm_forward"), x, options, out);
+  });
+}
+
+int qh_hip_bpm_vjp_tables(const float* x, const qh_c64* r, const qh_bpm_table_view* tables,
+                          const qh_bpm_options* options, float* grad) {
+  return guarded([&] {
+    require(x && r && grad, "bpm_vjp: null buffer");
+    bpm_vjp_call(tables_from_view(tables, options, "bpm_vjp"), x, r, options, grad);
+  });
+}
+
+int qh_hip_bpm_jvp_tables(const float* x, const float* h, const qh_bpm_table_view* tables,
+                          const qh_bpm_options* options, qh_c64* out) {
+  return guarded([&] {
+    require(x && h && out, "bpm_jvp: null buffer");
+    bpm_jvp_call(tables_from_view(tables, options, "bpm_jvp"), x, h, options, out);
+  });
+}
+
+int qh_hip_bpm_reconstruct_tables(const qh_c64* fields, float* x,
+                                  const qh_bpm_table_view* tables, const qh_bpm_options* options,
+                                  const qh_bpm_solve_options* solve, double* loss) {
+  return guarded([&] {
+    require(fields && x && solve && (loss || solve->iterations == 0), "bpm_reconstruct: null buffer");
+    const BpmTables t = tables_from_view(tables, options, "bpm_reconstruct");
+    check_solve_options(solve, t.n, "bpm_reconstruct");
+    require(t.lipschitz > 0, "bpm_reconstruct: the tables' Lipschitz constant must be positive");
+    bpm_reconstruct_call(t, fields, x, options, solve, loss);
   });
 }
 
@@ -890,6 +1001,14 @@ int qh_hip_bpm_vjp_dev(qh_bpm_plan* plan, const float* x, const qh_c64* r, float
   return guarded([&] {
     require(plan && x && r && grad, "bpm_vjp_dev: null argument");
     plan->impl->vjp(x, reinterpret_cast<const float2*>(r), grad, plan_stream(stream));
+  });
+}
+
+int qh_hip_bpm_jvp_dev(qh_bpm_plan* plan, const float* x, const float* h, qh_c64* out,
+                       void* stream) {
+  return guarded([&] {
+    require(plan && x && h && out, "bpm_jvp_dev: null argument");
+    plan->impl->jvp(x, h, reinterpret_cast<float2*>(out), plan_stream(stream));
   });
 }
 

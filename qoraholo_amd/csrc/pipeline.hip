@@ -192,44 +192,49 @@ void Pipeline::process_host(const float* user_in, float* user_out) {
   timings_.total = since_ms(t0);
 }
 
+bool Pipeline::ensure_graph(const float* holo_dev) {
+  if (graph_.ready() && graph_input_ == holo_dev) return true;
+  // Captured on the pipeline's own stream (a caller's stream may be the null stream, which
+  // cannot be captured), without the stage marks: events recorded inside a capture only
+  // order work within the graph and are not time-stamped by its replays.
+  // A capture can be invalidated by work other threads start meanwhile (allocations and
+  // synchronous copies on the null stream); the pipeline then stays with kernel-by-kernel
+  // launches, whose results are the same.
+  const hipStream_t cs = compute_.get();
+  check(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+  capturing_ = true;
+  try {
+    front_end(holo_dev, cs, false);
+    solver_.run(fields_.get(), cs, nullptr, 0);
+    capturing_ = false;
+    graph_.end_capture(cs);
+    graph_input_ = holo_dev;
+    return true;
+  } catch (const DeviceError&) {
+    capturing_ = false;
+    GraphExec::abandon_capture(cs);
+    graph_.reset();
+    use_graph_ = false;
+    return false;
+  } catch (...) {
+    capturing_ = false;
+    GraphExec::abandon_capture(cs);
+    graph_.reset();
+    throw;
+  }
+}
+
+void Pipeline::capture() {
+  if (!has_background_) throw LogicError("Pipeline: set_background() first");
+  if (use_graph_ && !solver_.timing_fgp()) ensure_graph(holo_.get());
+}
+
 void Pipeline::compute(const float* holo_dev, hipStream_t s) {
-  graph_ran_ = use_graph_ && !solver_.timing_fgp();
+  graph_ran_ = use_graph_ && !solver_.timing_fgp() && ensure_graph(holo_dev);
   if (!graph_ran_) {
     front_end(holo_dev, s, false);
     solver_.run(fields_.get(), s, &clock_, kSolver);
     return;
-  }
-  if (!graph_.ready() || graph_input_ != holo_dev) {
-    // Captured on the pipeline's own stream (a caller's stream may be the null stream, which
-    // cannot be captured), without the stage marks: events recorded inside a capture only
-    // order work within the graph and are not time-stamped by its replays.
-    // A capture can be invalidated by work other threads start meanwhile (allocations and
-    // synchronous copies on the null stream); the pipeline then stays with kernel-by-kernel
-    // launches, whose results are the same.
-    const hipStream_t cs = compute_.get();
-    check(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-    capturing_ = true;
-    try {
-      front_end(holo_dev, cs, false);
-      solver_.run(fields_.get(), cs, nullptr, 0);
-      capturing_ = false;
-      graph_.end_capture(cs);
-      graph_input_ = holo_dev;
-    } catch (const DeviceError&) {
-      capturing_ = false;
-      GraphExec::abandon_capture(cs);
-      graph_.reset();
-      use_graph_ = false;
-      graph_ran_ = false;
-      front_end(holo_dev, s, false);
-      solver_.run(fields_.get(), s, &clock_, kSolver);
-      return;
-    } catch (...) {
-      capturing_ = false;
-      GraphExec::abandon_capture(cs);
-      graph_.reset();
-      throw;
-    }
   }
   clock_.mark(kStart, s);
   graph_.launch(s);

@@ -43,6 +43,11 @@ class Pipeline {
     use_graph_ = on;
     if (!on) graph_.reset();
   }
+  // Captures the graph of a request on the pipeline's own device input now, so that the first
+  // process() or run() only replays it (a server builds its slots this way, under the lock that
+  // also serialises the allocations of other slots). No device work runs.
+  void capture();
+  bool graph_ready() const { return graph_.ready(); }
   bool has_background() const { return has_background_; }
   void set_setup(double ms, std::size_t bytes) {
     setup_ms_ = ms;
@@ -56,6 +61,9 @@ class Pipeline {
   // Queues the front end and the solver on s reading holograms at holo_dev: through the graph
   // captured for holo_dev (captured now if there is none), or kernel by kernel.
   void compute(const float* holo_dev, hipStream_t s);
+  // The graph captured for holo_dev (captured now if there is none); false when the capture
+  // failed and the pipeline fell back to kernel-by-kernel launches.
+  bool ensure_graph(const float* holo_dev);
   void collect(bool downloaded);
   // Orders stream s after the work the last process_device() queued on its caller's stream:
   // every request shares the pipeline's buffers, whatever stream it runs on.

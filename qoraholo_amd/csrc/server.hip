@@ -28,16 +28,18 @@ class ReconWorker final : public SlotWorker {
     std::vector<float> backgrounds;
   };
   explicit ReconWorker(const Spec& spec) : pipeline_(spec.options) {
-    // Slots launch kernel by kernel: a graph capture on one slot's thread would be invalidated
-    // by another slot being set up (allocations, synchronous copies) meanwhile, and with
-    // several slots in flight the launch gaps are covered by the other slots' work.
-    pipeline_.use_graph(false);
     pipeline_.set_background(spec.backgrounds.data());
+    // Slots are built under the server lock, which also covers every allocation and free of
+    // other slots, so no other slot's setup can invalidate this capture; every request of the
+    // slot then replays the graph (the slot falls back to kernel-by-kernel launches, with the
+    // same results, if the capture fails).
+    pipeline_.capture();
   }
   void run(const void* input, float* out, double*) override {
     pipeline_.run(static_cast<const float*>(input), out);
   }
   double filled() const override { return pipeline_.filled(); }
+  bool replays_graph() const override { return pipeline_.graph_ready(); }
 
  private:
   Pipeline pipeline_;
@@ -353,12 +355,14 @@ qh_server_stats Server::stats() {
   s.recon_per_s = span > 0 ? 1000.0 * static_cast<double>(completed_) / span : 0.0;
   s.device_bytes = used_bytes_;
   s.budget_bytes = budget_;
-  int live = 0, busy = 0;
+  int live = 0, busy = 0, graphs = 0;
   for (auto& sl : slots_) {
     live += sl->pipeline ? 1 : 0;
     busy += sl->busy ? 1 : 0;
+    graphs += sl->pipeline && sl->pipeline->replays_graph() ? 1 : 0;
   }
   s.slots = live;
+  s.graph_slots = graphs;
   s.busy_slots = busy;
   s.peak_busy_slots = peak_busy_;
   return s;

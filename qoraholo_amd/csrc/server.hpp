@@ -35,6 +35,8 @@ class SlotWorker {
   virtual void run(const void* input, float* out, double* extra) = 0;
   // Filled fraction of the last reconstruction's k-space (0 for other kinds).
   virtual double filled() const { return 0.0; }
+  // The slot's requests replay a captured graph of their device work.
+  virtual bool replays_graph() const { return false; }
 };
 
 class Server {

@@ -82,6 +82,80 @@ def test_vjp_invariant_to_checkpoints_and_chunks(device, qh):
             checkpoint, chunk)
 
 
+@pytest.mark.parametrize("nz,n,count", [(16, 32, 5), (24, 48, 7), (12, 64, 4), (8, 96, 3),
+                                         (6, 256, 2)])
+def test_jvp_matches_oracle(device, qh, oracle, nz, n, count):
+    """Device tangent model (bpm_jvp, src/cpu/bpm.cpp:141-176) against the float oracle's and
+    against the FP64 one, where the float oracle sets the tolerance: as close as the float
+    oracle is, within a factor 2."""
+    g, il, _, x = case(qh, nz, n, count=count)
+    rng = np.random.default_rng(nz * n)
+    h = rng.standard_normal(x.shape).astype(np.float32)
+    print(f"bpm jvp nz={nz} n={n} count={count}", flush=True)
+    got = qh.bpm_jvp(x, h, il, g)
+    f32 = oracle.bpm_jvp(x, h, il, GEOM)
+    f64 = oracle.bpm_jvp(x, h, il, GEOM, dtype=np.float64)
+    assert got.shape == f64.shape
+    assert rel_l2(got, f32) < 2e-5
+    assert rel_l2(got, f64) < max(2 * rel_l2(f32, f64), 1e-5)
+    # Chunked the same bits.
+    assert np.array_equal(qh.bpm_jvp(x, h, il, g, chunk=1), got)
+
+
+def test_jvp_adjoint_identity_and_difference_on_device(device, qh):
+    """<J h, r> = <h, J^T r> with both sides from the device, and J h against a central
+    difference of the device forward model (the checks of the reference's
+    tests/test_bpm.cpp:88-126); the difference at eps 1e-3 is float-limited, the reference
+    documents 1.75e-4 for its CPU model."""
+    g, il, _, x = case(qh, 16, 32, count=5)
+    rng = np.random.default_rng(5)
+    h = rng.standard_normal(x.shape).astype(np.float32)
+    r = residual_pattern(il.shape[0], 32)
+    jh = qh.bpm_jvp(x, h, il, g).astype(np.complex128)
+    lhs = np.sum(np.real(np.conj(jh) * r.astype(np.complex128)))
+    rhs = np.sum(h.astype(np.float64) * qh.bpm_vjp(x, r, il, g).astype(np.float64))
+    assert abs(lhs - rhs) / abs(rhs) < 2e-5
+    eps = np.float32(1e-3)
+    from qoraholo_amd.bpm import bpm_forward_contrast
+
+    fp = bpm_forward_contrast(x + eps * h, il, g).astype(np.complex128)
+    fm = bpm_forward_contrast(x - eps * h, il, g).astype(np.complex128)
+    fd = (fp - fm) / (2 * 1e-3)
+    assert rel_l2(fd, jh) < 2e-3
+
+
+def test_table_entry_points_equal_geometry_entry_points(device, qh):
+    """The calls on BpmTables (the reference's backend signature, src/backends.hpp:119-124)
+    give the same bits as the calls that build the tables from the illuminations."""
+    g, il, ri, x = case(qh, 16, 48, count=5)
+    t = qh.BpmTables.build(48, 16, il, g)
+    r = residual_pattern(il.shape[0], 48)
+    rng = np.random.default_rng(9)
+    h = rng.standard_normal(x.shape).astype(np.float32)
+    assert np.array_equal(t.forward(x), qh.bpm_forward(ri, il, g))
+    assert np.array_equal(t.vjp(x, r), qh.bpm_vjp(x, r, il, g))
+    assert np.array_equal(t.jvp(x, h), qh.bpm_jvp(x, h, il, g))
+    fields = qh.bpm_forward(ri, il, g)
+    start = np.full_like(x, np.float32(0.0))
+    got, loss = t.reconstruct(fields, start, iterations=3, tv_weight=1e-2)
+    want, want_loss = qh.bpm_reconstruct(fields, il, g, start + np.float32(GEOM[1]), iterations=3,
+                                         tv_weight=1e-2, return_loss=True)
+    assert np.array_equal(got + np.float32(GEOM[1]), want)
+    assert np.array_equal(loss, want_loss)
+
+
+def test_jvp_dev_equals_host_call(device, qh):
+    import torch
+
+    g, il, _, x = case(qh, 16, 32, count=5)
+    rng = np.random.default_rng(6)
+    h = rng.standard_normal(x.shape).astype(np.float32)
+    plan = qh.BpmPlan(16, 32, il, g)
+    got = plan.jvp(torch.from_numpy(x).cuda(), torch.from_numpy(h).cuda())
+    torch.cuda.synchronize()
+    assert np.array_equal(got.cpu().numpy(), qh.bpm_jvp(x, h, il, g))
+
+
 def test_adjoint_identity_on_device(device, qh, oracle):
     """<J h, r> with the FP64 tangent model against <h, J^T r> from the device adjoint."""
     g, il, _, x = case(qh, 16, 32, count=5)

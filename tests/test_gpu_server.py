@@ -46,6 +46,8 @@ def test_results_are_the_same_alone_and_in_the_mix(device, qh, mixed, max_slots)
     assert 1 <= s["peak_busy_slots"] <= max_slots
     # Idle slots stay resident: each of the three geometries keeps one (the budget is ample).
     assert s["slots"] >= len(mixed)
+    # Every reconstruction slot captured its graph when it was built and replays it.
+    assert s["graph_slots"] == s["slots"]
 
 
 def test_submissions_from_several_threads(device, qh, mixed):
