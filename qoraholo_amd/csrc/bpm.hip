@@ -107,6 +107,25 @@ namespace {
 
 using Args = BpmModel::Args;
 
+// Side of the lines of engine E: a compile-time constant for the register engines, so that the
+// line arithmetic below compiles to multiplications; the LDS engine's lines have any length.
+template <class E>
+__device__ __forceinline__ int line_side(int n) {
+  if constexpr (requires { E::N; }) {
+    return E::N;
+  } else {
+    return n;
+  }
+}
+
+// Row (or column) r of line l = b n + r within its illumination's plane, in 32-bit arithmetic
+// (a chunk holds far fewer than 2^31 lines; 64-bit remainders by a runtime n had been expanded
+// once per element).
+template <class E>
+__device__ __forceinline__ int line_in_plane(long l, int n) {
+  return static_cast<int>(static_cast<unsigned>(l) % static_cast<unsigned>(line_side<E>(n)));
+}
+
 // Rows (line l = b n + y, element i at l n + i): inverse x transform of T, envelope, screen,
 // forward x transform. in == null: the unit envelope of the first slice. out == null: store the
 // envelope only.
@@ -117,7 +136,7 @@ __global__ void __launch_bounds__(E::kBlock) k_bpm_rows_fwd(Args a) {
   if (a.in == nullptr) {
     e.load([&](long l, int i) {
       if (a.w) a.w[l * n + i] = float2{1.0f, 0.0f};
-      return a.screen[(l % n) * n + i];
+      return a.screen[line_in_plane<E>(l, n) * n + i];
     });
     if (a.out == nullptr) return;
     e.first(false);
@@ -129,7 +148,7 @@ __global__ void __launch_bounds__(E::kBlock) k_bpm_rows_fwd(Args a) {
   e.mid([&](long l, int k, float2& v) {
     const float2 w = cscale(v, a.inv);
     if (a.w) a.w[l * n + k] = w;
-    v = cmul(a.screen[(l % n) * n + k], w);
+    v = cmul(a.screen[line_in_plane<E>(l, n) * n + k], w);
   });
   if (a.out == nullptr) return;
   e.second(false);
@@ -146,9 +165,9 @@ __global__ void __launch_bounds__(E::kBlock) k_bpm_rows_jvp(Args a) {
   // The complex operations of the reference, in its order: (0, phase h) w, + dw, t times that.
   const auto tangent = [&](long l, int k, float2 dw) {
     const float2 w = a.w[l * n + k];
-    const float p = a.phase * a.h[(l % n) * n + k];
+    const float p = a.phase * a.h[line_in_plane<E>(l, n) * n + k];
     const float2 d{dw.x + (0.0f * w.x - p * w.y), dw.y + (0.0f * w.y + p * w.x)};
-    return cmul(a.screen[(l % n) * n + k], d);
+    return cmul(a.screen[line_in_plane<E>(l, n) * n + k], d);
   };
   if (a.in == nullptr) {
     e.load([&](long l, int i) { return tangent(l, i, float2{0.0f, 0.0f}); });
@@ -163,8 +182,12 @@ __global__ void __launch_bounds__(E::kBlock) k_bpm_rows_jvp(Args a) {
   e.store2([&](long l, int m, float2 v) { a.out[l * n + m] = v; });
 }
 
+template <class E>
 __device__ __forceinline__ long col_at(long l, int i, int n, long plane) {
-  return (l / n) * plane + static_cast<long>(i) * n + (l % n);
+  const unsigned m = static_cast<unsigned>(line_side<E>(n));
+  const unsigned lu = static_cast<unsigned>(l);
+  const unsigned b = lu / m;
+  return static_cast<long>(b) * plane + static_cast<long>(i) * m + (lu - b * m);
 }
 
 // Columns (line l = b n + kx, element i at b n^2 + i n + kx): forward y transform, propagation
@@ -174,17 +197,17 @@ __global__ void __launch_bounds__(E::kBlock) k_bpm_cols_fwd(Args a) {
   E e(a.g);
   const int n = a.g.n;
   const long plane = static_cast<long>(n) * n;
-  e.load([&](long l, int i) { return a.in[col_at(l, i, n, plane)]; });
+  e.load([&](long l, int i) { return a.in[col_at<E>(l, i, n, plane)]; });
   e.first(false);
   e.mid([&](long l, int ky, float2& v) {
-    const long x = col_at(l, ky, n, plane);
+    const long x = col_at<E>(l, ky, n, plane);
     float2 s;
     if (a.s_in) {
       s = a.s_in[x];
     } else if (a.s_zero) {
       s = float2{0.0f, 0.0f};
     } else {
-      s = (ky == 0 && l % n == 0) ? float2{static_cast<float>(plane), 0.0f} : float2{0.0f, 0.0f};
+      s = (ky == 0 && line_in_plane<E>(l, n) == 0) ? float2{static_cast<float>(plane), 0.0f} : float2{0.0f, 0.0f};
     }
     const float2 c = a.c[x];
     bpm_step(c.x, c.y, a.q[x], v.x, v.y, s.x, s.y);
@@ -193,7 +216,7 @@ __global__ void __launch_bounds__(E::kBlock) k_bpm_cols_fwd(Args a) {
   });
   if (a.out == nullptr) return;
   e.second(true);
-  e.store2([&](long l, int m, float2 v) { a.out[col_at(l, m, n, plane)] = v; });
+  e.store2([&](long l, int m, float2 v) { a.out[col_at<E>(l, m, n, plane)] = v; });
 }
 
 // Rows after the last slice: y = inverse x transform / n^2. Without data, y is the output;
@@ -233,10 +256,10 @@ __global__ void __launch_bounds__(E::kBlock) k_bpm_cols_adj(Args a) {
   E e(a.g);
   const int n = a.g.n;
   const long plane = static_cast<long>(n) * n;
-  e.load([&](long l, int i) { return a.in[col_at(l, i, n, plane)]; });
+  e.load([&](long l, int i) { return a.in[col_at<E>(l, i, n, plane)]; });
   e.first(false);
   e.mid([&](long l, int ky, float2& v) {
-    const long x = col_at(l, ky, n, plane);
+    const long x = col_at<E>(l, ky, n, plane);
     float gre = v.x;
     float gim = v.y;
     if (!a.first) {
@@ -252,7 +275,7 @@ __global__ void __launch_bounds__(E::kBlock) k_bpm_cols_adj(Args a) {
     v = float2{are, aim};
   });
   e.second(true);
-  e.store2([&](long l, int m, float2 v) { a.out[col_at(l, m, n, plane)] = v; });
+  e.store2([&](long l, int m, float2 v) { a.out[col_at<E>(l, m, n, plane)] = v; });
 }
 
 // Adjoint rows: b = inverse x transform / n^2; gradient term phase Im(conj(t w) b) per
@@ -265,7 +288,7 @@ __global__ void __launch_bounds__(E::kBlock) k_bpm_rows_adj(Args a) {
   e.first(true);
   e.mid([&](long l, int k, float2& v) {
     const float2 b = cscale(v, a.inv);
-    const float2 t = a.screen[(l % n) * n + k];
+    const float2 t = a.screen[line_in_plane<E>(l, n) * n + k];
     const float2 tw = cmul(t, a.w[l * n + k]);
     a.p[l * n + k] = a.phase * (tw.x * b.y - tw.y * b.x);
     v = cmul(cconj(t), b);
@@ -408,6 +431,7 @@ BpmModel::BpmModel(const BpmTables& t, int chunk, int checkpoint)
       phase_(t.phase), lipschitz_(t.lipschitz) {
   require(fft_supported(n_), "bpm: n must factor into 2, 3 and 5 and be at most 2048");
   require(chunk >= 0 && checkpoint >= 0, "bpm: chunk and checkpoint must be non-negative");
+  require(static_cast<long>(t.count) * n_ < (1L << 31), "bpm: too many lines for one plan");
   seg_len_ = checkpoint == 0 || checkpoint > nz_ ? nz_ : checkpoint;
   nseg_ = (nz_ + seg_len_ - 1) / seg_len_;
   if (chunk == 0) {

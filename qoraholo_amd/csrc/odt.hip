@@ -26,13 +26,26 @@ __global__ void k_clear(float2* potential, std::uint16_t* counts, long total) {
   counts[i] = 0;
 }
 
-__global__ void k_ewald_gather(const float2* spectra, const float4* illum, int count, OdtGrid g,
-                               float2* potential, std::uint16_t* counts) {
+// A thread's column is zero before its first contribution (k_clear), so the first value a
+// voxel receives is written without reading the voxel back; only a voxel hit again (another
+// illumination landing on the same axial bin) is read, added to and written. A bit per axial
+// bin, in LDS, records which voxels of the column already hold a value. The spectra loads do
+// not depend on earlier iterations, so several are in flight at once (with the stores to the
+// volume, which they do not alias).
+__global__ void __launch_bounds__(64) k_ewald_gather(const float2* __restrict__ spectra,
+                                                     const float4* __restrict__ illum, int count,
+                                                     OdtGrid g, float2* __restrict__ potential,
+                                                     std::uint16_t* __restrict__ counts) {
+  extern __shared__ unsigned seen[];  // (nz + 31) / 32 words per thread, word-major
+  const int t = static_cast<int>(threadIdx.x);
+  const int words = (g.nz + 31) >> 5;
+  for (int w = 0; w < words; ++w) seen[w * 64 + t] = 0u;
   const long plane = static_cast<long>(g.n) * g.n;
-  const long col = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const long col = static_cast<long>(blockIdx.x) * blockDim.x + t;
   if (col >= plane) return;
   const int ky = static_cast<int>(col / g.n);
   const int kx = static_cast<int>(col % g.n);
+#pragma unroll 4
   for (int a = 0; a < count; ++a) {
     const float4 il = illum[a];
     int kz = 0;
@@ -43,11 +56,19 @@ __global__ void k_ewald_gather(const float2* spectra, const float4* illum, int c
     const float b = -g.gain * nuz;
     const float sign = parity(kz);
     const long i = kz * plane + col;
-    float2 p = potential[i];
+    unsigned& word = seen[(kz >> 5) * 64 + t];
+    const unsigned bit = 1u << (kz & 31);
+    float2 p{0.0f, 0.0f};
+    unsigned c = 0;
+    if (word & bit) {
+      p = potential[i];
+      c = counts[i];
+    }
+    word |= bit;
     p.x += sign * (0.0f * s.x - b * s.y);
     p.y += sign * (0.0f * s.y + b * s.x);
     potential[i] = p;
-    counts[i] = static_cast<std::uint16_t>(counts[i] + 1);
+    counts[i] = static_cast<std::uint16_t>(c + 1);
   }
 }
 
@@ -80,9 +101,13 @@ __global__ void k_fold_half(const float2* potential, const std::uint16_t* counts
   const long half = static_cast<long>(nz) * n * h;
   const long o = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (o < half) {
-    const int kx = static_cast<int>(o % h);
-    const int ky = static_cast<int>((o / h) % n);
-    const int kz = static_cast<int>(o / (static_cast<long>(h) * n));
+    // 32-bit index arithmetic (a half spectrum has fewer than 2^31 bins): a 64-bit division
+    // per bin had cost more than the bin's loads.
+    const unsigned row = static_cast<unsigned>(o) / static_cast<unsigned>(h);
+    const int kx = static_cast<int>(static_cast<unsigned>(o) - row * static_cast<unsigned>(h));
+    const unsigned zq = row / static_cast<unsigned>(n);
+    const int ky = static_cast<int>(row - zq * static_cast<unsigned>(n));
+    const int kz = static_cast<int>(zq);
     const long i = (static_cast<long>(kz) * n + ky) * n + kx;
     const long j = (static_cast<long>(negative_bin(kz, nz)) * n + negative_bin(ky, n)) * n +
                    negative_bin(kx, n);
@@ -134,6 +159,7 @@ OdtSolver::OdtSolver(const OdtGrid& g, int count, const OdtConfig& c)
       half_(static_cast<long>(g.nz) * g.n * (g.n / 2 + 1)) {
   require(fft_supported(g.n) && fft_supported(g.nz), "reconstruct: sizes must factor into 2, 3, 5");
   require(count <= 65535, "reconstruct: at most 65535 illuminations");
+  require(half_ < (1L << 31), "reconstruct: at most 2^31 half-spectrum bins");
   const float k0 = 2.0f * kPi / c.wavelength;
   nm2_ = c.n_medium * c.n_medium;
   inv_k02_ = 1.0f / (k0 * k0);
@@ -191,7 +217,8 @@ void OdtSolver::map(float2* fields, hipStream_t s, StageClock* clock, int first,
   check_launch("clear");
   fill(reinterpret_cast<float*>(filled_.get()), 2, 0.0f, s);
   mark(kClear);
-  hipLaunchKernelGGL(k_ewald_gather, dim3(blocks_for(plane, 64)), dim3(64), 0, s, fields,
+  const std::size_t seen_bytes = static_cast<std::size_t>((g_.nz + 31) / 32) * 64 * sizeof(unsigned);
+  hipLaunchKernelGGL(k_ewald_gather, dim3(blocks_for(plane, 64)), dim3(64), seen_bytes, s, fields,
                      illum_.get(), count_, g_, potential_.get(), counts_.get());
   check_launch("ewald_gather");
   mark(kMapping);
@@ -221,6 +248,7 @@ void OdtSolver::run(float2* fields, hipStream_t s, StageClock* clock, int first)
     if (clock) clock->mark(first + m, s);
   };
   map(fields, s, clock, first, real_);
+  float* out = out_ ? out_ : ri_.get();
   const int n = g_.n;
   const int nz = g_.nz;
   const int h = n / 2 + 1;
@@ -256,7 +284,7 @@ void OdtSolver::run(float2* fields, hipStream_t s, StageClock* clock, int first)
       // Gerchberg-Papoulis (src/cpu/odt.cpp:76-90): x = max(x, 0), R2C, restore, C2R / N.
       for (int it = 0; it < c_.iterations; ++it) {
         const bool last = it == c_.iterations - 1;
-        gradient_step(x_[0].get(), nullptr, RealIn::nonneg, 0.0f, last ? ri_.get() : x_[0].get(),
+        gradient_step(x_[0].get(), nullptr, RealIn::nonneg, 0.0f, last ? out : x_[0].get(),
                       last ? RealOut::index : RealOut::scale);
       }
       mark(kSolve);
@@ -274,7 +302,7 @@ void OdtSolver::run(float2* fields, hipStream_t s, StageClock* clock, int first)
       const bool last = it == c_.iterations - 1;
       gradient_step(x_[cur].get(), x_[prev].get(), RealIn::momentum, m, v, RealOut::scale);
       const int next = prev != cur ? prev : 1 - cur;
-      tv_state_->prox(v, last ? ri_.get() : x_[next].get(), lambda_, c_.tv_inner, s, last, nm2_,
+      tv_state_->prox(v, last ? out : x_[next].get(), lambda_, c_.tv_inner, s, last, nm2_,
                       inv_k02_, time_fgp_ ? &fgp_events_[2 * it] : nullptr,
                       time_fgp_ ? &fgp_events_[2 * it + 1] : nullptr);
       prev = cur;
@@ -311,7 +339,7 @@ void OdtSolver::run(float2* fields, hipStream_t s, StageClock* clock, int first)
     fft3(p, nz, n, n, 0b111u, true, inv, s);
   }
   mark(kSolve);
-  hipLaunchKernelGGL(k_index_complex, dim3(blocks_for(vol_)), dim3(256), 0, s, p, ri_.get(), vol_,
+  hipLaunchKernelGGL(k_index_complex, dim3(blocks_for(vol_)), dim3(256), 0, s, p, out, vol_,
                      nm2_, inv_k02_);
   check_launch("index");
   mark(kFinish);

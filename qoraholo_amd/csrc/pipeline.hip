@@ -153,7 +153,7 @@ void Pipeline::process_host(const float* user_in, float* user_out) {
     }
     chunk_done_.back().record(c);
     check(hipStreamWaitEvent(s, chunk_done_.back().get(), 0), "hipStreamWaitEvent");
-    compute(holo_.get(), s);
+    compute(holo_.get(), nullptr, s);
   } else {
     graph_ran_ = false;
     front_end(nullptr, s, true, user_in);
@@ -192,8 +192,8 @@ void Pipeline::process_host(const float* user_in, float* user_out) {
   timings_.total = since_ms(t0);
 }
 
-bool Pipeline::ensure_graph(const float* holo_dev) {
-  if (graph_.ready() && graph_input_ == holo_dev) return true;
+bool Pipeline::ensure_graph(const float* holo_dev, float* ri_dev) {
+  if (graph_.ready() && graph_input_ == holo_dev && graph_output_ == ri_dev) return true;
   // Captured on the pipeline's own stream (a caller's stream may be the null stream, which
   // cannot be captured), without the stage marks: events recorded inside a capture only
   // order work within the graph and are not time-stamped by its replays.
@@ -205,19 +205,24 @@ bool Pipeline::ensure_graph(const float* holo_dev) {
   capturing_ = true;
   try {
     front_end(holo_dev, cs, false);
+    solver_.set_output(ri_dev);
     solver_.run(fields_.get(), cs, nullptr, 0);
+    solver_.set_output(nullptr);
     capturing_ = false;
     graph_.end_capture(cs);
     graph_input_ = holo_dev;
+    graph_output_ = ri_dev;
     return true;
   } catch (const DeviceError&) {
     capturing_ = false;
+    solver_.set_output(nullptr);
     GraphExec::abandon_capture(cs);
     graph_.reset();
     use_graph_ = false;
     return false;
   } catch (...) {
     capturing_ = false;
+    solver_.set_output(nullptr);
     GraphExec::abandon_capture(cs);
     graph_.reset();
     throw;
@@ -226,14 +231,16 @@ bool Pipeline::ensure_graph(const float* holo_dev) {
 
 void Pipeline::capture() {
   if (!has_background_) throw LogicError("Pipeline: set_background() first");
-  if (use_graph_ && !solver_.timing_fgp()) ensure_graph(holo_.get());
+  if (use_graph_ && !solver_.timing_fgp()) ensure_graph(holo_.get(), nullptr);
 }
 
-void Pipeline::compute(const float* holo_dev, hipStream_t s) {
-  graph_ran_ = use_graph_ && !solver_.timing_fgp() && ensure_graph(holo_dev);
+void Pipeline::compute(const float* holo_dev, float* ri_dev, hipStream_t s) {
+  graph_ran_ = use_graph_ && !solver_.timing_fgp() && ensure_graph(holo_dev, ri_dev);
   if (!graph_ran_) {
     front_end(holo_dev, s, false);
+    solver_.set_output(ri_dev);
     solver_.run(fields_.get(), s, &clock_, kSolver);
+    solver_.set_output(nullptr);
     return;
   }
   clock_.mark(kStart, s);
@@ -244,10 +251,8 @@ void Pipeline::compute(const float* holo_dev, hipStream_t s) {
 void Pipeline::process_device(const float* holo_dev, float* ri_dev, hipStream_t s) {
   if (!has_background_) throw LogicError("Pipeline: set_background() first");
   after_device_work(s);
-  compute(holo_dev, s);
-  const std::size_t vol = static_cast<std::size_t>(solver_.nz()) * solver_.n() * solver_.n();
-  check(hipMemcpyAsync(ri_dev, solver_.index(), vol * sizeof(float), hipMemcpyDeviceToDevice, s),
-        "copy index");
+  // The index volume is written straight into ri_dev (the graph is captured for that address).
+  compute(holo_dev, ri_dev, s);
   check(hipMemcpyAsync(filled_.get(), solver_.filled_device(), sizeof(unsigned long long),
                        hipMemcpyDeviceToHost, s),
         "download filled");

@@ -59,11 +59,13 @@ class Pipeline {
   void front_end(const float* holo_dev, hipStream_t s, bool from_host, const float* user = nullptr);
   void process_host(const float* user_in, float* user_out);
   // Queues the front end and the solver on s reading holograms at holo_dev: through the graph
-  // captured for holo_dev (captured now if there is none), or kernel by kernel.
-  void compute(const float* holo_dev, hipStream_t s);
-  // The graph captured for holo_dev (captured now if there is none); false when the capture
-  // failed and the pipeline fell back to kernel-by-kernel launches.
-  bool ensure_graph(const float* holo_dev);
+  // captured for holo_dev and ri_dev (captured now if there is none), or kernel by kernel.
+  // ri_dev: where the index volume goes (null: the solver's own buffer, which run() and
+  // process() download from).
+  void compute(const float* holo_dev, float* ri_dev, hipStream_t s);
+  // The graph captured for holo_dev and ri_dev (captured now if there is none); false when the
+  // capture failed and the pipeline fell back to kernel-by-kernel launches.
+  bool ensure_graph(const float* holo_dev, float* ri_dev);
   void collect(bool downloaded);
   // Orders stream s after the work the last process_device() queued on its caller's stream:
   // every request shares the pipeline's buffers, whatever stream it runs on.
@@ -91,6 +93,7 @@ class Pipeline {
   bool device_pending_ = false;
   GraphExec graph_;
   const float* graph_input_ = nullptr;
+  float* graph_output_ = nullptr;
   bool use_graph_ = true;
   bool graph_ran_ = false;  // the last request ran through the graph
   bool capturing_ = false;  // stage marks are skipped while the graph is captured

@@ -112,6 +112,9 @@ class OdtSolver {
   // Normalised k-space potential and hit counts after the mapping (for intermediate checks).
   void map_only(float2* fields, hipStream_t s);
   const float* index() const { return ri_.get(); }
+  // Where the next run() writes the index volume: a caller's device buffer of (nz, n, n)
+  // floats, or null for index().
+  void set_output(float* out) { out_ = out; }
   const float2* potential() const { return potential_.get(); }
   const std::uint16_t* counts() const { return counts_.get(); }
   const unsigned long long* filled_device() const { return filled_.get(); }
@@ -146,6 +149,7 @@ class OdtSolver {
   DeviceBuffer<float2> e_;
   DeviceBuffer<float> x_[3];
   DeviceBuffer<float> ri_;
+  float* out_ = nullptr;
   DeviceBuffer<unsigned long long> filled_;
   std::unique_ptr<TvState> tv_state_;
   std::vector<Event> fgp_events_;
