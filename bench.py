@@ -200,6 +200,34 @@ def cpu_baseline(w, runs=3):
     }, ri
 
 
+def streaming_reference(torch, mib=256, reps=20):
+    """What plain streaming kernels reach on this box, beside the roofline's 8 TB/s: torch's
+    device copy (read 1, write 1) and its vectorised addcmul (read 3 float4 streams, write 1) on
+    `mib` MiB float32 tensors, timed with events over `reps` launches each."""
+    n = mib * 1024 * 1024 // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    b = torch.full_like(a, 2.0)
+    c = torch.full_like(a, 3.0)
+    d = torch.empty_like(a)
+    out = {"tensor_bytes": n * 4}
+    for name, fn, moved in (("copy", lambda: d.copy_(a), 2),
+                            ("addcmul", lambda: torch.addcmul(a, b, c, out=d), 4)):
+        fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        out[f"{name}_gbs"] = round(moved * n * 4 / (ms * 1e-3) / 1e9, 1)
+    del a, b, c, d
+    torch.cuda.empty_cache()
+    return out
+
+
 def fgp_traffic_run(qh, torch) -> int:
     """One TV prox of 10 FGP steps on a 256^3 volume (five two-step passes, plus the prox's fill
     and primal kernels) and one copy of a 256 MiB buffer (reads and writes exactly 256 MiB) as
@@ -811,6 +839,7 @@ def main() -> int:
         # mean launch duration.
         roofline["traffic_gbs"] = round(roofline["traffic"] / (fgp_avg_ms * 1e-3) / 1e9, 1)
         roofline["traffic_frac"] = round(roofline["traffic_gbs"] / HBM_PEAK_GBS, 4)
+    roofline["streaming_reference"] = streaming_reference(torch)
     recon_bytes = reconstruction_bytes(ANGLES, 2 * N, N, N, ITERATIONS, INNER)
     per_recon = {
         "bytes": recon_bytes,

@@ -72,18 +72,38 @@ __global__ void __launch_bounds__(64) k_ewald_gather(const float2* __restrict__ 
   }
 }
 
+// Blocks of the counting kernels: 8 per CU of 256 threads, a few thousand global atomics.
+constexpr unsigned kCountBlocks = 2048;
+
+// Adds c (0, 1 or 2) of every lane of the wave to *count with one LDS atomic per wave (one per
+// lane serialised the whole block on a single address). All lanes of the wave call it.
+__device__ __forceinline__ void add_wave_count(unsigned int* count, unsigned c) {
+  const unsigned long long one = __ballot(c >= 1u);
+  const unsigned long long two = __ballot(c >= 2u);
+  if ((threadIdx.x & 63u) == 0) {
+    const unsigned w = static_cast<unsigned>(__popcll(one) + __popcll(two));
+    if (w) atomicAdd(count, w);
+  }
+}
+
 // Divides every voxel with hits by its count and counts the filled voxels.
 __global__ void k_normalize(float2* potential, const std::uint16_t* counts, long total,
                             unsigned long long* filled) {
   __shared__ unsigned int block_count;
   if (threadIdx.x == 0) block_count = 0;
   __syncthreads();
-  const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < total && counts[i] > 0) {
-    const float w = static_cast<float>(counts[i]);
-    float2 p = potential[i];
-    potential[i] = {p.x / w, p.y / w};
-    atomicAdd(&block_count, 1u);
+  // Grid-stride: a bounded grid keeps the global atomics (one per block, on a single address)
+  // few; one per block of 256 voxels had serialised tens of thousands of them.
+  const long stride = static_cast<long>(gridDim.x) * blockDim.x;
+  for (long base = static_cast<long>(blockIdx.x) * blockDim.x; base < total; base += stride) {
+    const long i = base + threadIdx.x;
+    const bool hit = i < total && counts[i] > 0;
+    if (hit) {
+      const float w = static_cast<float>(counts[i]);
+      float2 p = potential[i];
+      potential[i] = {p.x / w, p.y / w};
+    }
+    add_wave_count(&block_count, hit ? 1u : 0u);
   }
   __syncthreads();
   if (threadIdx.x == 0 && block_count > 0) atomicAdd(filled, static_cast<unsigned long long>(block_count));
@@ -99,7 +119,11 @@ __global__ void k_fold_half(const float2* potential, const std::uint16_t* counts
   __syncthreads();
   const int h = n / 2 + 1;
   const long half = static_cast<long>(nz) * n * h;
-  const long o = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  // Grid-stride over the half spectrum (see k_normalize).
+  const long stride = static_cast<long>(gridDim.x) * blockDim.x;
+  for (long base = static_cast<long>(blockIdx.x) * blockDim.x; base < half; base += stride) {
+  const long o = base + threadIdx.x;
+  unsigned c = 0;
   if (o < half) {
     // 32-bit index arithmetic (a half spectrum has fewer than 2^31 bins): a 64-bit division
     // per bin had cost more than the bin's loads.
@@ -121,9 +145,10 @@ __global__ void k_fold_half(const float2* potential, const std::uint16_t* counts
     hermitian_data(pi.x, pi.y, pj.x, pj.y, v.x, v.y);
     a[o] = v;
     hits[o] = static_cast<std::uint8_t>((wi > 0) + (wj > 0));
-    unsigned c = wi > 0 ? 1u : 0u;
+    c = wi > 0 ? 1u : 0u;
     if (kx >= 1 && kx <= n / 2 - 1 && wj > 0) ++c;
-    if (c) atomicAdd(&block_count, c);
+  }
+  add_wave_count(&block_count, c);
   }
   __syncthreads();
   if (threadIdx.x == 0 && block_count > 0) atomicAdd(filled, static_cast<unsigned long long>(block_count));
@@ -223,11 +248,13 @@ void OdtSolver::map(float2* fields, hipStream_t s, StageClock* clock, int first,
   check_launch("ewald_gather");
   mark(kMapping);
   if (fold) {
-    hipLaunchKernelGGL(k_fold_half, dim3(blocks_for(half_)), dim3(256), 0, s, potential_.get(),
+    hipLaunchKernelGGL(k_fold_half, dim3(std::min(blocks_for(half_), kCountBlocks)), dim3(256), 0, s,
+                       potential_.get(),
                        counts_.get(), g_.nz, g_.n, a_.get(), hits_.get(), filled_.get());
     check_launch("fold_half");
   } else {
-    hipLaunchKernelGGL(k_normalize, dim3(blocks_for(vol_)), dim3(256), 0, s, potential_.get(),
+    hipLaunchKernelGGL(k_normalize, dim3(std::min(blocks_for(vol_), kCountBlocks)), dim3(256), 0, s,
+                       potential_.get(),
                        counts_.get(), vol_, filled_.get());
     check_launch("normalize");
   }
