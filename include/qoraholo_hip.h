@@ -264,6 +264,8 @@ typedef struct {
   int busy_slots;      /* slots running a request now */
   int peak_busy_slots; /* most slots running requests at once since the last reset */
   int graph_slots;     /* resident slots whose requests replay one captured hipGraph */
+  int64_t batched_requests; /* requests that ran together with others of their geometry */
+  int peak_batch;      /* most requests one slot ran at once since the last reset */
 } qh_server_stats;
 
 /* budget_bytes 0: 90% of the free device memory. max_slots: requests in flight at once; idle
@@ -274,6 +276,12 @@ void qh_hip_server_destroy(qh_server* s);
 /* A geometry (the options of a Pipeline) and its background holograms (count, N, N). */
 int qh_hip_server_add_geometry(qh_server* s, const qh_pipeline_options* options,
                                const float* backgrounds, int* geometry);
+/* The same with batched slots: a slot of the geometry takes up to max_batch (1 to 64) queued
+ * requests of it at once, runs the front end of all their hologram stacks in one set of
+ * launches and their solves in turn, as one captured graph per batch size; each result equals
+ * the request's alone bit for bit. */
+int qh_hip_server_add_geometry_batched(qh_server* s, const qh_pipeline_options* options,
+                                       const float* backgrounds, int max_batch, int* geometry);
 /* Queues holograms (count, N, N) of a geometry; the result (nz, n, n) goes to ri_out. Both
  * host buffers must stay valid until qh_hip_server_wait returns for the id. Thread-safe. */
 int qh_hip_server_submit(qh_server* s, int geometry, const float* holograms, float* ri_out,

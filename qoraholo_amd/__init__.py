@@ -523,10 +523,12 @@ class Server:
                      geometry: Geometry, backgrounds, radius: float = 0.0, output_size: int = 0,
                      nz: int = 0, iterations: int = 0, solver: str = "gp",
                      gp_transform: str = "r2c", tv_weight: float = 1e-2, tv_inner: int = 10,
-                     mapping: str = "gather") -> int:
+                     mapping: str = "gather", max_batch: int = 1) -> int:
         """Registers a geometry. With ``center`` or ``illuminations`` None they are estimated
         from the backgrounds on the device (``estimate_illuminations``; the hologram pixel is
-        ``geometry.pixel * output_size / hologram_size``)."""
+        ``geometry.pixel * output_size / hologram_size``). With ``max_batch`` > 1 a slot of the
+        geometry takes up to that many queued requests of it at once (one front end for all
+        their stacks, one graph replay for the batch); results are the same bits."""
         if center is None or illuminations is None:
             m = output_size if output_size > 0 else hologram_size
             hg = Geometry(geometry.wavelength, geometry.n_medium, geometry.na,
@@ -541,8 +543,12 @@ class Server:
         if b.shape != (count, hologram_size, hologram_size):
             raise ValueError("Server: backgrounds must be (count, N, N)")
         gid = ctypes.c_int()
-        _native.check(_native.lib().qh_hip_server_add_geometry(self._handle, ctypes.byref(o),
-                                                               _native.ptr(b), ctypes.byref(gid)))
+        if max_batch > 1:
+            _native.check(_native.lib().qh_hip_server_add_geometry_batched(
+                self._handle, ctypes.byref(o), _native.ptr(b), int(max_batch), ctypes.byref(gid)))
+        else:
+            _native.check(_native.lib().qh_hip_server_add_geometry(
+                self._handle, ctypes.byref(o), _native.ptr(b), ctypes.byref(gid)))
         m = o.crop.out
         self._geometries[gid.value] = ((count, hologram_size, hologram_size),
                                        (nz if nz > 0 else m, m, m), None)

@@ -62,7 +62,8 @@ class ServerStats_t(ctypes.Structure):
                 ("failed", ctypes.c_int64), ("queued", ctypes.c_int64), ("p50_ms", c_double),
                 ("p99_ms", c_double), ("mean_ms", c_double), ("recon_per_s", c_double),
                 ("device_bytes", c_size_t), ("budget_bytes", c_size_t), ("slots", c_int),
-                ("busy_slots", c_int), ("peak_busy_slots", c_int), ("graph_slots", c_int)]
+                ("busy_slots", c_int), ("peak_busy_slots", c_int), ("graph_slots", c_int),
+                ("batched_requests", ctypes.c_int64), ("peak_batch", c_int)]
 
 
 class BpmOptions_t(ctypes.Structure):
@@ -130,6 +131,8 @@ _SIGNATURES = {
     "qh_hip_server_destroy": (None, [c_void_p]),
     "qh_hip_server_add_geometry": (c_int, [c_void_p, POINTER(PipelineOptions_t), c_void_p,
                                            POINTER(c_int)]),
+    "qh_hip_server_add_geometry_batched": (c_int, [c_void_p, POINTER(PipelineOptions_t), c_void_p,
+                                                   c_int, POINTER(c_int)]),
     "qh_hip_server_submit": (c_int, [c_void_p, c_int, c_void_p, c_void_p, POINTER(ctypes.c_int64)]),
     "qh_hip_server_add_bpm": (c_int, [c_void_p, POINTER(BpmRequestOptions_t), POINTER(c_int)]),
     "qh_hip_server_submit_bpm": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p,

@@ -699,6 +699,14 @@ int qh_hip_server_add_geometry(qh_server* s, const qh_pipeline_options* options,
   });
 }
 
+int qh_hip_server_add_geometry_batched(qh_server* s, const qh_pipeline_options* options,
+                                       const float* backgrounds, int max_batch, int* geometry) {
+  return guarded([&] {
+    require(s && options && backgrounds && geometry, "Server: null argument");
+    *geometry = s->impl->add_geometry(*options, backgrounds, max_batch);
+  });
+}
+
 int qh_hip_server_submit(qh_server* s, int geometry, const float* holograms, float* ri_out,
                          int64_t* id) {
   return guarded([&] {

@@ -42,12 +42,14 @@ __global__ void k_crop(const float2* half, int count, int n, int m, int cy, int 
   out[i] = v;
 }
 
-__global__ void k_ratio_phase(const float2* s, const float2* b, long total, float* psi,
-                              float* logamp) {
+// b holds `period` values, repeated over the samples (one background stack for several
+// requests); 32-bit indices (a batch of stacks holds fewer than 2^31 pixels).
+__global__ void k_ratio_phase(const float2* s, const float2* b, long total, unsigned period,
+                              float* psi, float* logamp) {
   const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= total) return;
   const float2 u = s[i];
-  const float2 v = b[i];
+  const float2 v = b[static_cast<unsigned>(i) % period];
   const float d = 1.0f / (v.x * v.x + v.y * v.y);
   const float re = (u.x * v.x + u.y * v.y) * d;
   const float im = (u.y * v.x - u.x * v.y) * d;
@@ -167,9 +169,12 @@ void fields_inverse(float2* fields, int count, int m, float scale, bool demodula
 }
 
 void ratio_phase(const float2* sample, const float2* background, long total, float* psi,
-                 float* logamp, hipStream_t s) {
+                 float* logamp, hipStream_t s, long period) {
+  if (period == 0) period = total;
+  require(total < (1L << 31) && period >= 1 && total % period == 0,
+          "ratio_phase: the samples must be whole background stacks");
   hipLaunchKernelGGL(k_ratio_phase, dim3(blocks_for(total)), dim3(256), 0, s, sample, background,
-                     total, psi, logamp);
+                     total, static_cast<unsigned>(period), psi, logamp);
   check_launch("ratio_phase");
 }
 

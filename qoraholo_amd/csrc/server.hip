@@ -1,6 +1,7 @@
 // Served reconstruction loop (see server.hpp).
 #include <algorithm>
 
+#include "recon_batch.hpp"
 #include "server.hpp"
 
 namespace qh {
@@ -45,6 +46,28 @@ class ReconWorker final : public SlotWorker {
   Pipeline pipeline_;
 };
 
+// A reconstruction slot that takes up to `capacity` requests of its geometry at once.
+class ReconBatchWorker final : public SlotWorker {
+ public:
+  ReconBatchWorker(const ReconWorker::Spec& spec, int capacity)
+      : batch_(spec.options, spec.backgrounds.data(), capacity) {}
+  void run(const void* input, float* out, double*) override {
+    const float* in = static_cast<const float*>(input);
+    batch_.run(&in, &out, 1);
+  }
+  void run_batch(const void* const* inputs, float* const* outs, double* const*, int nb) override {
+    std::vector<const float*> in(nb);
+    for (int b = 0; b < nb; ++b) in[b] = static_cast<const float*>(inputs[b]);
+    batch_.run(in.data(), outs, nb);
+  }
+  double filled() const override { return batch_.filled(); }
+  bool replays_graph() const override { return batch_.graphs_ready(); }
+  int capacity() const override { return batch_.capacity(); }
+
+ private:
+  ReconBatch batch_;
+};
+
 // A BPM inversion slot.
 class BpmWorker final : public SlotWorker {
  public:
@@ -87,18 +110,26 @@ void Server::reset_stats() {
   std::lock_guard<std::mutex> lock(mu_);
   latencies_.clear();
   submitted_ = completed_ = failed_ = 0;
+  batched_ = 0;
+  peak_batch_ = 0;
   peak_busy_ = busy_slots();
 }
 
-int Server::add_geometry(const qh_pipeline_options& o, const float* backgrounds) {
+int Server::add_geometry(const qh_pipeline_options& o, const float* backgrounds, int max_batch) {
   check_pipeline_options(o);
   require(backgrounds != nullptr, "Server: background holograms are required");
+  require(max_batch >= 1 && max_batch <= 64, "Server: max_batch must be between 1 and 64");
   auto spec = std::make_shared<ReconWorker::Spec>();
   spec->options = o;
   spec->illuminations.assign(o.illuminations, o.illuminations + o.count);
   spec->options.illuminations = spec->illuminations.data();
   const std::size_t image = static_cast<std::size_t>(o.hologram_size) * o.hologram_size;
   spec->backgrounds.assign(backgrounds, backgrounds + image * o.count);
+  if (max_batch > 1) {
+    return add_kind(kReconstruction, [spec, max_batch]() -> std::unique_ptr<SlotWorker> {
+      return std::make_unique<ReconBatchWorker>(*spec, max_batch);
+    });
+  }
   return add_kind(kReconstruction, [spec] { return std::make_unique<ReconWorker>(*spec); });
 }
 
@@ -202,25 +233,17 @@ std::int64_t Server::submit(int geometry, Kind kind, const void* input, float* r
   return r->id;
 }
 
-// Called with mu_ held: an idle slot of the request's geometry, or a new slot when the budget
-// and the slot limit allow one (evicting idle slots of other geometries if needed).
-bool Server::try_admit(const std::shared_ptr<Request>& r) {
+Server::Slot* Server::slot_for(const std::shared_ptr<Request>& r) {
   // max_slots bounds the requests in flight; idle slots stay resident (background fields, plans
   // and solver state of their geometry) until the HBM budget needs their memory.
-  if (busy_slots() >= max_slots_) return false;
+  if (busy_slots() >= max_slots_) return nullptr;
   for (auto& s : slots_) {
-    if (!s->busy && !s->stop && s->geometry == r->geometry && s->pipeline) {
-      s->busy = true;
-      peak_busy_ = std::max(peak_busy_, busy_slots());
-      s->work = r;
-      s->cv.notify_all();
-      return true;
-    }
+    if (!s->busy && !s->stop && s->geometry == r->geometry && s->pipeline) return s.get();
   }
   Geometry& g = *geometries_[r->geometry];
   while (used_bytes_ + g.slot_bytes > budget_) {
     // Evict one idle slot of another geometry; stop when none is idle.
-    if (!evict_idle(r->geometry)) return false;
+    if (!evict_idle(r->geometry)) return nullptr;
   }
   auto slot = std::make_unique<Slot>();
   slot->geometry = r->geometry;
@@ -228,39 +251,58 @@ bool Server::try_admit(const std::shared_ptr<Request>& r) {
   slot->bytes = g.slot_bytes;
   used_bytes_ += slot->bytes;
   ++g.slots;
-  slot->busy = true;
-  slot->work = r;
   Slot* raw = slot.get();
   slots_.push_back(std::move(slot));
-  peak_busy_ = std::max(peak_busy_, busy_slots());
   raw->thread = std::thread([this, raw] { slot_loop(raw); });
-  return true;
+  return raw;
 }
 
 void Server::dispatch_loop() {
   std::unique_lock<std::mutex> lock(mu_);
   while (!stop_) {
     bool progress = false;
-    for (auto it = queue_.begin(); it != queue_.end();) {
-      bool admitted = false;
+    // Indexes, not iterators: erasing inside a deque invalidates its iterators.
+    for (std::size_t i = 0; i < queue_.size();) {
+      Slot* slot = nullptr;
       try {
-        admitted = try_admit(*it);
+        slot = slot_for(queue_[i]);
       } catch (const std::exception& e) {
         // A slot that cannot be built fails this request only.
-        auto& r = **it;
+        auto& r = *queue_[i];
         r.finished = true;
         r.status = dynamic_cast<const OutOfMemory*>(&e) ? QH_ERR_OUT_OF_MEMORY : QH_ERR_RUNTIME;
         r.error = e.what();
         ++failed_;
         done_cv_.notify_all();
-        admitted = true;
-      }
-      if (admitted) {
-        it = queue_.erase(it);
+        queue_.erase(queue_.begin() + static_cast<std::ptrdiff_t>(i));
         progress = true;
-      } else {
-        ++it;
+        continue;
       }
+      if (!slot) {
+        ++i;
+        continue;
+      }
+      // The slot takes this request and, up to its capacity, the later queued requests of the
+      // same geometry, in arrival order.
+      const int geometry = queue_[i]->geometry;
+      const std::size_t cap = static_cast<std::size_t>(slot->pipeline->capacity());
+      std::vector<std::shared_ptr<Request>> batch{queue_[i]};
+      queue_.erase(queue_.begin() + static_cast<std::ptrdiff_t>(i));
+      for (std::size_t j = i; j < queue_.size() && batch.size() < cap;) {
+        if (queue_[j]->geometry == geometry) {
+          batch.push_back(queue_[j]);
+          queue_.erase(queue_.begin() + static_cast<std::ptrdiff_t>(j));
+        } else {
+          ++j;
+        }
+      }
+      if (batch.size() > 1) batched_ += static_cast<std::int64_t>(batch.size());
+      peak_batch_ = std::max(peak_batch_, static_cast<int>(batch.size()));
+      slot->busy = true;
+      slot->work = std::move(batch);
+      peak_busy_ = std::max(peak_busy_, busy_slots());
+      slot->cv.notify_all();
+      progress = true;
     }
     if (!progress) queue_cv_.wait(lock);
   }
@@ -269,15 +311,28 @@ void Server::dispatch_loop() {
 void Server::slot_loop(Slot* slot) {
   std::unique_lock<std::mutex> lock(mu_);
   while (true) {
-    slot->cv.wait(lock, [&] { return slot->stop || slot->work; });
-    if (slot->stop && !slot->work) return;
-    std::shared_ptr<Request> r = slot->work;
+    slot->cv.wait(lock, [&] { return slot->stop || !slot->work.empty(); });
+    if (slot->stop && slot->work.empty()) return;
+    std::vector<std::shared_ptr<Request>> batch = slot->work;
     lock.unlock();
     int status = QH_OK;
     std::string error;
     double filled = 0;
     try {
-      slot->pipeline->run(r->input, r->out, r->extra);
+      if (batch.size() == 1) {
+        slot->pipeline->run(batch[0]->input, batch[0]->out, batch[0]->extra);
+      } else {
+        std::vector<const void*> in;
+        std::vector<float*> out;
+        std::vector<double*> extra;
+        for (const auto& r : batch) {
+          in.push_back(r->input);
+          out.push_back(r->out);
+          extra.push_back(r->extra);
+        }
+        slot->pipeline->run_batch(in.data(), out.data(), extra.data(),
+                                  static_cast<int>(batch.size()));
+      }
       filled = slot->pipeline->filled();
     } catch (const InvalidArgument& e) {
       status = QH_ERR_INVALID_ARGUMENT;
@@ -291,19 +346,21 @@ void Server::slot_loop(Slot* slot) {
     }
     const auto now = std::chrono::steady_clock::now();
     lock.lock();
-    r->finished = true;
-    r->status = status;
-    r->error = error;
-    r->filled = filled;
-    r->latency_ms = ms_between(r->arrival, now);
-    if (status == QH_OK) {
-      ++completed_;
-      latencies_.push_back(r->latency_ms);
-      last_done_ = now;
-    } else {
-      ++failed_;
+    for (const auto& r : batch) {
+      r->finished = true;
+      r->status = status;
+      r->error = error;
+      r->filled = filled;
+      r->latency_ms = ms_between(r->arrival, now);
+      if (status == QH_OK) {
+        ++completed_;
+        latencies_.push_back(r->latency_ms);
+        last_done_ = now;
+      } else {
+        ++failed_;
+      }
     }
-    slot->work.reset();
+    slot->work.clear();
     slot->busy = false;
     done_cv_.notify_all();
     queue_cv_.notify_all();
@@ -363,6 +420,8 @@ qh_server_stats Server::stats() {
   }
   s.slots = live;
   s.graph_slots = graphs;
+  s.batched_requests = batched_;
+  s.peak_batch = peak_batch_;
   s.busy_slots = busy;
   s.peak_busy_slots = peak_busy_;
   return s;

@@ -37,6 +37,12 @@ class SlotWorker {
   virtual double filled() const { return 0.0; }
   // The slot's requests replay a captured graph of their device work.
   virtual bool replays_graph() const { return false; }
+  // Requests of its geometry the slot runs together; run_batch runs nb <= capacity() of them.
+  virtual int capacity() const { return 1; }
+  virtual void run_batch(const void* const* inputs, float* const* outs, double* const* extras,
+                         int nb) {
+    for (int b = 0; b < nb; ++b) run(inputs[b], outs[b], extras[b]);
+  }
 };
 
 class Server {
@@ -47,8 +53,9 @@ class Server {
   Server& operator=(const Server&) = delete;
 
   // Registers a reconstruction geometry with its background holograms (count, N, N); returns
-  // its id.
-  int add_geometry(const qh_pipeline_options& o, const float* backgrounds);
+  // its id. max_batch > 1: its slots take up to that many queued requests of the geometry at
+  // once (one front end for all their stacks, one graph replay for the batch).
+  int add_geometry(const qh_pipeline_options& o, const float* backgrounds, int max_batch = 1);
   // Registers a BPM inversion geometry; returns its id.
   int add_bpm(const BpmInversionSpec& spec);
   // Queues a request; its input (holograms (count, N, N) of a reconstruction geometry, field
@@ -90,7 +97,7 @@ class Server {
     int geometry;
     std::unique_ptr<SlotWorker> pipeline;
     std::size_t bytes = 0;
-    std::shared_ptr<Request> work;
+    std::vector<std::shared_ptr<Request>> work;  // the requests it runs now (a batch)
     bool busy = false;
     bool stop = false;
     std::condition_variable cv;
@@ -104,7 +111,10 @@ class Server {
   bool evict_idle(int keep);
   void dispatch_loop();
   void slot_loop(Slot* slot);
-  bool try_admit(const std::shared_ptr<Request>& r);
+  // Called with mu_ held: an idle slot of the request's geometry, or a new one when the budget
+  // and the slot limit allow (evicting idle slots of other geometries if needed); null when the
+  // request must wait.
+  Slot* slot_for(const std::shared_ptr<Request>& r);
 
   std::size_t budget_;
   int max_slots_;
@@ -118,6 +128,8 @@ class Server {
   std::vector<std::unique_ptr<Slot>> retired_;
   std::size_t used_bytes_ = 0;
   int peak_busy_ = 0;  // most slots running requests at once since the last reset_stats
+  std::int64_t batched_ = 0;  // requests that ran in a batch of two or more
+  int peak_batch_ = 0;
   std::int64_t next_id_ = 1;
   bool stop_ = false;
   std::vector<double> latencies_;

@@ -34,9 +34,10 @@ void crop_fields(const float2* half, int count, int n, const CropPlan& p, float2
 // Inverse 2D transform of (count, m, m) fields times scale, optionally demodulated by
 // (-1)^(x+y) (the centre-origin output of the per-call path).
 void fields_inverse(float2* fields, int count, int m, float scale, bool demodulate, hipStream_t s);
-// psi = arg(s / b), logamp = log|s / b| with r = s conj(b) / |b|^2.
+// psi = arg(s / b), logamp = log|s / b| with r = s conj(b) / |b|^2. The background holds
+// `period` values (0: total), repeated over the samples: one stack for several requests.
 void ratio_phase(const float2* sample, const float2* background, long total, float* psi,
-                 float* logamp, hipStream_t s);
+                 float* logamp, hipStream_t s, long period = 0);
 // psi = arg r, logamp = log|r| of a given ratio.
 void split_ratio(const float2* ratio, long total, float* psi, float* logamp, hipStream_t s);
 // Sideband peak of each hologram of a stack (src/cpu/field.cpp:35-49 on the device): integer
@@ -59,14 +60,15 @@ void fft2_images(float2* data, int count, int n, bool inverse, hipStream_t s);
 class Unwrapper {
  public:
   Unwrapper(int count, int ny, int nx);
-  // Least-squares unwrap of psi into phi, congruent to psi modulo 2 pi.
-  void unwrap(const float* psi, float* phi, hipStream_t s);
+  // Least-squares unwrap of psi into phi, congruent to psi modulo 2 pi. `images` (0: count)
+  // takes the first images of the capacity; each image is solved on its own.
+  void unwrap(const float* psi, float* phi, hipStream_t s, int images = 0);
   // out = logamp + i (unwrap(psi) - 2 pi round(mean over the border / 2 pi)).
-  void rytov(const float* psi, const float* logamp, float2* out, hipStream_t s);
+  void rytov(const float* psi, const float* logamp, float2* out, hipStream_t s, int images = 0);
   std::size_t bytes() const;
 
  private:
-  void solve(const float* psi, hipStream_t s, bool border);
+  void solve(const float* psi, hipStream_t s, bool border, int images);
   int count_, ny_, nx_;
   int nblk_;
   DeviceBuffer<float> phi_;

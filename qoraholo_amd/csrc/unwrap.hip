@@ -269,16 +269,17 @@ std::size_t Unwrapper::bytes() const {
   return phi_.bytes() + partial_.bytes() + shift_.bytes() + eig_y_.bytes() + eig_x_.bytes();
 }
 
-void Unwrapper::solve(const float* psi, hipStream_t s, bool border) {
+void Unwrapper::solve(const float* psi, hipStream_t s, bool border, int images) {
+  require(images >= 1 && images <= count_, "unwrap: image count beyond the capacity");
   const long plane = static_cast<long>(ny_) * nx_;
-  const long total = plane * count_;
+  const long total = plane * images;
   float* phi = phi_.get();
   hipLaunchKernelGGL(k_divergence, dim3(blocks_for(total)), dim3(256), 0, s, psi, phi, total, ny_, nx_);
   check_launch("divergence");
   for (int pass = 0; pass < 2; ++pass) {
     const bool forward = pass == 0;
     if (!forward) {
-      ColArgs c{phi, static_cast<long>(count_) * nx_, ny_, nx_,
+      ColArgs c{phi, static_cast<long>(images) * nx_, ny_, nx_,
                 std::max(1, std::min(16, 2048 / ny_)), fft_factors(ny_), fft_twiddles(ny_),
                 dct_twiddles(ny_), eig_y_.get(), eig_x_.get(), 1.0f / static_cast<float>(ny_)};
       const std::size_t lds = (2 * ny_ + 2 * static_cast<std::size_t>(c.lines_per_block) * ny_) * sizeof(float2);
@@ -286,7 +287,7 @@ void Unwrapper::solve(const float* psi, hipStream_t s, bool border) {
       hipLaunchKernelGGL(k_dct_cols_poisson, dim3(grid), dim3(kThreads), lds, s, c);
       check_launch("dct_cols_poisson");
     }
-    DctArgs d{phi, static_cast<long>(count_) * ny_, nx_, rows_per_block(nx_), fft_factors(nx_),
+    DctArgs d{phi, static_cast<long>(images) * ny_, nx_, rows_per_block(nx_), fft_factors(nx_),
               fft_twiddles(nx_), dct_twiddles(nx_), 1.0f / static_cast<float>(nx_)};
     const std::size_t lds = (2 * nx_ + 2 * static_cast<std::size_t>(d.rows_per_block) * nx_) * sizeof(float2);
     const unsigned grid = static_cast<unsigned>((d.rows + d.rows_per_block - 1) / d.rows_per_block);
@@ -297,27 +298,30 @@ void Unwrapper::solve(const float* psi, hipStream_t s, bool border) {
     }
     check_launch("dct_rows");
   }
-  hipLaunchKernelGGL(k_congruence_partial, dim3(nblk_, count_), dim3(kThreads), 0, s, psi, phi,
+  hipLaunchKernelGGL(k_congruence_partial, dim3(nblk_, images), dim3(kThreads), 0, s, psi, phi,
                      plane, partial_.get());
   check_launch("congruence_partial");
-  hipLaunchKernelGGL(k_congruence_finish, dim3(count_), dim3(kThreads), 0, s, partial_.get(),
+  hipLaunchKernelGGL(k_congruence_finish, dim3(images), dim3(kThreads), 0, s, partial_.get(),
                      nblk_, phi, ny_, nx_, border ? 1 : 0, shift_.get());
   check_launch("congruence_finish");
 }
 
-void Unwrapper::unwrap(const float* psi, float* out, hipStream_t s) {
-  solve(psi, s, false);
+void Unwrapper::unwrap(const float* psi, float* out, hipStream_t s, int images) {
+  if (images == 0) images = count_;
+  solve(psi, s, false, images);
   const long plane = static_cast<long>(ny_) * nx_;
-  const long total = plane * count_;
+  const long total = plane * images;
   hipLaunchKernelGGL(k_add_congruence, dim3(blocks_for(total)), dim3(256), 0, s, phi_.get(),
                      shift_.get(), plane, total, out);
   check_launch("add_congruence");
 }
 
-void Unwrapper::rytov(const float* psi, const float* logamp, float2* out, hipStream_t s) {
-  solve(psi, s, true);
+void Unwrapper::rytov(const float* psi, const float* logamp, float2* out, hipStream_t s,
+                      int images) {
+  if (images == 0) images = count_;
+  solve(psi, s, true, images);
   const long plane = static_cast<long>(ny_) * nx_;
-  const long total = plane * count_;
+  const long total = plane * images;
   hipLaunchKernelGGL(k_assemble_rytov, dim3(blocks_for(total)), dim3(256), 0, s, logamp,
                      phi_.get(), shift_.get(), plane, total, out);
   check_launch("assemble_rytov");

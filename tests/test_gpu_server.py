@@ -215,3 +215,52 @@ def test_tight_budget_evicts_across_request_kinds(device, qh, mixed, bpm_mixed):
             assert np.array_equal(ri, mixed[0][1])
         s = server.stats()
         assert s["device_bytes"] <= budget and s["slots"] == 1
+
+
+@pytest.mark.parametrize("solver,iterations", [("gp", 5), ("tv", 3)])
+def test_batched_slots_give_each_request_its_alone_result(device, qh, solver, iterations):
+    """A geometry with batched slots: queued requests of it run together (one front end for
+    all their stacks, one graph replay per batch) and each gets the bits it gets alone on a
+    Pipeline, whatever batch it lands in; distinct inputs show the results are routed to their
+    own requests."""
+    d = PipelineData(qh, holo=128, angles=9, iterations=iterations, solver=solver)
+    inputs = [(d.holo * np.float32(1.0 + 0.02 * k)).astype(np.float32) for k in range(3)]
+    p = qh.Pipeline(d.count, d.size, illuminations=d.illum, geometry=d.geom, **d.kw)
+    p.set_background(d.bg)
+    alone = [p.run(x) for x in inputs]
+    filled = p.filled
+    del p
+    assert not np.array_equal(alone[0], alone[1])
+    server = qh.Server(max_slots=1)
+    gid = server.add_geometry(d.count, d.size, illuminations=d.illum, geometry=d.geom,
+                              backgrounds=d.bg, max_batch=4, **d.kw)
+    order = [0, 1, 2, 2, 0, 1, 1, 2, 0, 0]
+    print(f"batched slots {solver}: {len(order)} requests", flush=True)
+    ids = [(k, server.submit(gid, inputs[k])) for k in order]
+    for k, rid in ids:
+        ri, got_filled, latency = server.wait(rid)
+        assert np.array_equal(ri, alone[k]), k
+        assert got_filled == filled
+        assert latency > 0
+    s = server.stats()
+    assert s["completed"] == len(order) and s["failed"] == 0
+    assert 2 <= s["peak_batch"] <= 4
+    assert s["batched_requests"] >= 2
+    assert s["graph_slots"] == s["slots"]
+
+
+def test_batched_and_single_geometries_mix(device, qh, mixed):
+    """A batched geometry beside unbatched ones, interleaved submissions: every result is the
+    request's alone result."""
+    server = qh.Server(max_slots=2)
+    d0, alone0 = mixed[0]
+    g0 = server.add_geometry(d0.count, d0.size, illuminations=d0.illum, geometry=d0.geom,
+                             backgrounds=d0.bg, max_batch=3, **d0.kw)
+    rest = add_all(server, mixed[1:])
+    gids = [g0] + rest
+    order = [0, 0, 1, 0, 2, 0, 0, 1, 0]
+    ids = [(k, server.submit(gids[k], mixed[k][0].holo)) for k in order]
+    for k, rid in ids:
+        ri, _, _ = server.wait(rid)
+        assert np.array_equal(ri, mixed[k][1]), k
+    assert server.stats()["failed"] == 0
