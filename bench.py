@@ -228,16 +228,16 @@ def streaming_reference(torch, mib=256, reps=20):
     return out
 
 
-def fgp_traffic_run(qh, torch) -> int:
-    """One TV prox of 10 FGP steps on a 256^3 volume (five two-step passes, plus the prox's fill
+def fgp_traffic_run(qh, torch, n: int = N) -> int:
+    """One TV prox of 10 FGP steps on an n^3 volume (five two-step passes, plus the prox's fill
     and primal kernels) and one copy of a 256 MiB buffer (reads and writes exactly 256 MiB) as
     the counter calibration."""
     from qoraholo_amd import _native
 
     rng = np.random.default_rng(0)
-    b = (rng.standard_normal((N, N, N)) * 0.01).astype(np.float32)
+    b = (rng.standard_normal((n, n, n)) * 0.01).astype(np.float32)
     x = np.empty_like(b)
-    _native.check(_native.lib().qh_hip_tv_prox(_native.ptr(b), N, N, N, 0.5, 10, 1,
+    _native.check(_native.lib().qh_hip_tv_prox(_native.ptr(b), n, n, n, 0.5, 10, 1,
                                                 _native.ptr(x)))
     src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda:0")
     dst = torch.empty_like(src)
@@ -248,7 +248,7 @@ def fgp_traffic_run(qh, torch) -> int:
     b = torch.full_like(src, 3.0)
     torch.addcmul(src, a, b, out=dst)
     torch.cuda.synchronize()
-    log("fgp traffic run: 10 FGP steps in 5 passes at 256^3, a 268435456-byte copy and an "
+    log(f"fgp traffic run: 10 FGP steps in 5 passes at {n}^3, a 268435456-byte copy and an "
         "addcmul of three 268435456-byte inputs")
     return 0
 
@@ -721,8 +721,11 @@ def main() -> int:
                     help="skip the served 192^3 x 49 BPM inversions (configuration 5) in the "
                          "default run")
     ap.add_argument("--fgp-traffic", action="store_true",
-                    help="run only 5 FGP steps at 256^3 and a 256 MiB device copy, for the "
-                         "rocprofv3 FETCH_SIZE / WRITE_SIZE runs that calibrate HBM traffic")
+                    help="run only one TV prox (10 FGP steps in 5 passes) and two calibration "
+                         "kernels, for the rocprofv3 FETCH_SIZE / WRITE_SIZE runs that measure "
+                         "HBM traffic")
+    ap.add_argument("--fgp-traffic-size", type=int, default=N,
+                    help="volume side of the --fgp-traffic run (default 256)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the reconstruction path runs on one device and the "
@@ -736,7 +739,7 @@ def main() -> int:
     from qoraholo_amd import workloads
 
     if args.fgp_traffic:
-        return fgp_traffic_run(qh, torch)
+        return fgp_traffic_run(qh, torch, args.fgp_traffic_size)
 
     if not qh.hip_available():
         raise SystemExit("no HIP device or libqoraholo_hip.so missing")

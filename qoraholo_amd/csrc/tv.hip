@@ -302,13 +302,47 @@ struct PairArgs {
   int tiles_x;  // x tiles per row
 };
 
+// Whether column xb + v of a lane can lie on the volume's x faces. Rows that start at column 0
+// (not XNEG) with whole-vector access (VEC: nx a multiple of VX, xb a multiple of VX) put the
+// faces only in a lane's first (x = 0) and last (x = nx - 1) element, so the middle elements of
+// an active lane (xb < nx) never need the x tests.
+template <int VX, bool XNEG, bool VEC>
+__device__ __forceinline__ constexpr bool x_left_face(int v) {
+  return XNEG || !VEC || v == 0;
+}
+template <int VX, bool XNEG, bool VEC>
+__device__ __forceinline__ constexpr bool x_right_face(int v) {
+  return XNEG || !VEC || v == VX - 1;
+}
+
 // z = max(b - lambda L(r), 0) on one row of plane zp (the expression order of primal_plane).
-template <int VX, bool XNEG>
+// zp and y are the same for every lane of the wave: on planes and rows off the volume's z and y
+// faces every z and y term is taken, so that branch runs the same float operations without the
+// per-element selects (the kernel's time is set by its VALU issue rate, not only by its bytes).
+template <int VX, bool XNEG, bool VEC>
 __device__ __forceinline__ void pair_primal(const PairArgs& a, int zp, int y, int xb,
                                             const float (&R)[3][VX], const float (&rz_below)[VX],
                                             const float (&ry_up)[VX], const float (&B)[VX],
                                             float (&Z)[VX]) {
   const float left = __shfl_up(R[2][VX - 1], 1);
+  if (zp > 0 && zp < a.nz - 1 && y > 0 && y < a.ny - 1) {
+#pragma unroll
+    for (int v = 0; v < VX; ++v) {
+      const int x = xb + v;
+      float s = 0.0f;
+      s += R[0][v];
+      s -= rz_below[v];
+      s += R[1][v];
+      s -= ry_up[v];
+      if (!x_right_face<VX, XNEG, VEC>(v) || x < a.nx - 1) s += R[2][v];
+      if (!x_left_face<VX, XNEG, VEC>(v) || x > 0) s -= (v == 0 ? left : R[2][v - 1]);
+      // Without XNEG and with VEC every element of an active lane is inside the row; inactive
+      // lanes (xb >= nx) are never stored and feed no active lane's result.
+      Z[v] = (XNEG || !VEC) ? (((!XNEG || x >= 0) && x < a.nx) ? tv_primal(B[v], a.lambda, s) : 0.0f)
+                            : tv_primal(B[v], a.lambda, s);
+    }
+    return;
+  }
 #pragma unroll
   for (int v = 0; v < VX; ++v) {
     const int x = xb + v;
@@ -326,21 +360,30 @@ __device__ __forceinline__ void pair_primal(const PairArgs& a, int zp, int y, in
 }
 
 // p' = P(r + step g) on one row of plane zp, g from z of this plane (zc), the plane above (zn)
-// and the row below (zdn): the dual step of k_fgp_step.
-template <int VX>
+// and the row below (zdn): the dual step of k_fgp_step. Off the last plane and the last row
+// (wave-uniform) the z and y differences are taken without selects.
+template <int VX, bool XNEG, bool VEC>
 __device__ __forceinline__ void pair_dual(const PairArgs& a, int zp, int y, int xb,
                                           const float (&zc)[VX], const float (&zn)[VX],
                                           const float (&zdn)[VX], const float (&R)[3][VX],
                                           float (&P)[3][VX]) {
   const float right = __shfl_down(zc[0], 1);
+  const bool inner = zp < a.nz - 1 && y < a.ny - 1;
 #pragma unroll
   for (int v = 0; v < VX; ++v) {
     const int x = xb + v;
     const float c = zc[v];
     float g[3];
-    g[0] = zp < a.nz - 1 ? c - zn[v] : 0.0f;
-    g[1] = y < a.ny - 1 ? c - zdn[v] : 0.0f;
-    g[2] = x < a.nx - 1 ? c - (v == VX - 1 ? right : zc[v + 1]) : 0.0f;
+    if (inner) {
+      g[0] = c - zn[v];
+      g[1] = c - zdn[v];
+    } else {
+      g[0] = zp < a.nz - 1 ? c - zn[v] : 0.0f;
+      g[1] = y < a.ny - 1 ? c - zdn[v] : 0.0f;
+    }
+    g[2] = (!x_right_face<VX, XNEG, VEC>(v) || x < a.nx - 1)
+               ? c - (v == VX - 1 ? right : zc[v + 1])
+               : 0.0f;
     float p3[3] = {0.0f, 0.0f, 0.0f};
     float r3[3] = {R[0][v], R[1][v], R[2][v]};
     tv_dual_step(g, a.step, 0.0f, p3, r3);
@@ -425,7 +468,9 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
   bd.nz = a.nz;
   bd.ny = a.ny;
   bd.nx = a.nx;
-  const int w = static_cast<int>(threadIdx.x) >> 6;
+  // The wave index is uniform across the wave: as a scalar it keeps the row arithmetic, the
+  // row addresses and the y tests off the vector ALU.
+  const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
   const int lane = static_cast<int>(threadIdx.x) & 63;
   // Tiles of one z chunk are consecutive (x fastest, then y), so the neighbours that share halo
   // rows and columns run on the same XCD. Rows wider than a wave are cut into x tiles of xt
@@ -510,7 +555,7 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
   {
     float up[VX];
     lds_get<VX, VEC>(ry, wu, LW, xl, up);
-    pair_primal<VX, XT>(a, zs, y, xb, rk1, rz_m2, up, b1, zk1);
+    pair_primal<VX, XT, VEC>(a, zs, y, xb, rk1, rz_m2, up, b1, zk1);
   }
   lds_put<VX, VEC>(zk, w, LW, xl, zk1);
   float rq0[3][VX], zq0[VX], zdn_k[VX], zdn_q[VX];
@@ -550,13 +595,13 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
     if (need_zk) {
       float up[VX];
       lds_get<VX, VEC>(ry, wu, LW, xl, up);
-      pair_primal<VX, XT>(a, f, y, xb, rk2, rk1[0], up, b2, zk2);
+      pair_primal<VX, XT, VEC>(a, f, y, xb, rk2, rk1[0], up, b2, zk2);
     }
     // p_{k+1} and r_{k+1} on plane t + 1.
     float rq1[3][VX] = {};
     if (need_pk1) {
       float pq1[3][VX];
-      pair_dual<VX>(a, t + 1, y, xb, zk1, zk2, zdn_k, rk1, pq1);
+      pair_dual<VX, XT, VEC>(a, t + 1, y, xb, zk1, zk2, zdn_k, rk1, pq1);
       if (a.p_k1 && out_row && out_col && t + 1 >= z0 && t + 1 < z1) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) store_row<VX, VEC>(a.p_k1 + c * a.vol, bd, t + 1, y, xb, pq1[c]);
@@ -577,14 +622,14 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
     if (need_zq) {
       float up[VX];
       lds_get<VX, VEC>(rqy, wu, LW, xl, up);
-      pair_primal<VX, XT>(a, t + 1, y, xb, rq1, rq0[0], up, b1, zq1);
+      pair_primal<VX, XT, VEC>(a, t + 1, y, xb, rq1, rq0[0], up, b1, zq1);
     }
     lds_put<VX, VEC>(zq, w, LW, xl, zq1);
     __syncthreads();  // (3) z_{k+1} of plane t + 1 visible
     // p_{k+2} on plane t.
     if (t >= z0 && need_pk2) {
       float p2[3][VX];
-      pair_dual<VX>(a, t, y, xb, zq0, zq1, zdn_q, rq0, p2);
+      pair_dual<VX, XT, VEC>(a, t, y, xb, zq0, zq1, zdn_q, rq0, p2);
       if (out_row && out_col) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) store_row<VX, VEC>(a.p_k2 + c * a.vol, bd, t, y, xb, p2[c]);
