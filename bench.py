@@ -374,6 +374,9 @@ def bpm_leg(qh, torch, cpu=True, repeats=5):
     return out
 
 
+MIXED_BATCH = 4
+
+
 def served_mixed(qh, rates=(0.25, 0.5, 1.0, 2.0), per_rate=100, seed=0, max_slots=8):
     """Config 3: a served stream of mixed shapes and solvers at Poisson rates set against the
     plain path's measured capacity. Shapes uniform over 25 x 256^2 -> 128^3, 49 x 512^2 -> 256^3
@@ -422,13 +425,18 @@ def served_mixed(qh, rates=(0.25, 0.5, 1.0, 2.0), per_rate=100, seed=0, max_slot
     log(f"mixed stream: plain ms {plain_ms}, pipeline ms {pipe_ms}, capacity {capacity:.2f}/s")
 
     server = qh.Server(max_slots=max_slots)
+    # The 128^3 kinds (about 1.6 ms of device time each) get batched slots: a slot takes up to
+    # four queued requests of its kind at once; the larger kinds are device-bound and run alone.
     gids = [server.add_geometry(w.holograms.shape[0], w.holograms.shape[1], w.center,
                                 w.illuminations, w.geometry, w.backgrounds, radius=w.radius,
-                                output_size=w.field_size, **kw) for _, w, kw in kinds]
+                                output_size=w.field_size,
+                                max_batch=MIXED_BATCH if w.holograms.shape[1] <= 256 else 1, **kw)
+            for _, w, kw in kinds]
     for k, (_, w, _) in enumerate(kinds):
         server.wait(server.submit(gids[k], w.holograms))
     rng = np.random.default_rng(seed)
-    out = {"kinds": [k[0] for k in kinds], "plain_ms": {k: round(v, 2) for k, v in plain_ms.items()},
+    out = {"kinds": [k[0] for k in kinds], "max_batch_128": MIXED_BATCH,
+           "plain_ms": {k: round(v, 2) for k, v in plain_ms.items()},
            "pipeline_ms": {k: round(v, 2) for k, v in pipe_ms.items()},
            "plain_capacity_per_s": round(capacity, 3), "rates": []}
     for factor in rates:
@@ -452,6 +460,7 @@ def served_mixed(qh, rates=(0.25, 0.5, 1.0, 2.0), per_rate=100, seed=0, max_slot
                "recon_s": round(st["recon_per_s"], 3), "p50_ms": round(st["p50_ms"], 2),
                "p99_ms": round(st["p99_ms"], 2), "requests": per_rate, "admitted": st["completed"],
                "peak_queued": peak_queue, "slots": st["slots"],
+               "batched_requests": st["batched_requests"], "peak_batch": st["peak_batch"],
                "device_gb": round(st["device_bytes"] / 1e9, 2)}
         out["rates"].append(row)
         log(f"mixed stream at {factor}x: {row}")

@@ -281,9 +281,11 @@ __global__ void __launch_bounds__(E::kBlock) k_bpm_cols_adj(Args a) {
 }
 
 // Adjoint rows: b = inverse x transform / n^2; gradient term phase Im(conj(t w) b) per
-// illumination; back = conj(t) b and its forward x transform (skipped on slice 0).
+// illumination; back = conj(t) b and its forward x transform (skipped on slice 0). Four waves
+// per SIMD: the 384-point kernel otherwise takes 129 registers, one over the limit for four.
 template <class E>
-__global__ void __launch_bounds__(E::kBlock) k_bpm_rows_adj(Args a) {
+__global__ void __launch_bounds__(E::kBlock) __attribute__((amdgpu_waves_per_eu(4)))
+k_bpm_rows_adj(Args a) {
   E e(a.g);
   const int n = a.g.n;
   e.load([&](long l, int i) { return a.in[l * n + i]; });

@@ -306,7 +306,11 @@ __global__ void __launch_bounds__((Shape4<N1, N2, LINE_FAST>::BLOCK)) k_fft4s_li
   __shared__ float2 tw[S::N];
   __shared__ float2 t[t_size<N1, N2, S::LINES, LINE_FAST>()];
   const int tid = static_cast<int>(threadIdx.x);
-  for (int i = tid; i < S::N; i += S::BLOCK) tw[i] = a.tw[i];
+  if constexpr (LINE_FAST) {
+    for (int i = tid; i < S::N; i += S::BLOCK) tw[i] = a.tw[i];
+  } else {
+    fill_twiddle_matrix<N2, N1, N1>(tw, a.tw, 1, tid, S::BLOCK);
+  }
   const int line = LINE_FAST ? tid % S::LINES : tid / S::T;
   const int j = LINE_FAST ? tid / S::LINES : tid % S::T;
   const long l = xcd_block() * S::LINES + line;
@@ -320,7 +324,8 @@ __global__ void __launch_bounds__((Shape4<N1, N2, LINE_FAST>::BLOCK)) k_fft4s_li
                               : float2{0.0f, 0.0f};
   }
   __syncthreads();
-  fft_4step<N1, N2, S::LINES, LINE_FAST>(v, w, j, line, t, tw, a.inverse != 0);
+  fft_4step<N1, N2, S::LINES, LINE_FAST, LINE_FAST ? kTwPlain : kTwByK, N1>(v, w, j, line, t, tw,
+                                                                          a.inverse != 0);
   if (valid && j < N2) {
     const int lm = a.alt_mod > 0 ? static_cast<int>(l % a.alt_mod) : 0;
 #pragma unroll
@@ -397,7 +402,7 @@ __global__ void __launch_bounds__((Shape4<N1, N2>::BLOCK)) k_rfft4s_rows(RowsArg
   __shared__ float2 twn[M + 1];
   __shared__ float2 t[t_size<N1, N2, S::LINES, false>()];
   const int tid = static_cast<int>(threadIdx.x);
-  for (int i = tid; i < M; i += S::BLOCK) twm[i] = a.tw[2 * i];
+  fill_twiddle_matrix<N2, N1, N1>(twm, a.tw, 2, tid, S::BLOCK);  // W_M^t = W_n^(2t)
   for (int i = tid; i <= M; i += S::BLOCK) twn[i] = a.tw[i];
   const int line = tid / S::T;
   const int j = tid % S::T;
@@ -434,7 +439,7 @@ __global__ void __launch_bounds__((Shape4<N1, N2>::BLOCK)) k_rfft4s_rows(RowsArg
     }
   }
   __syncthreads();
-  fft_4step<N1, N2, S::LINES, false>(v, w, j, line, t, twm, false);
+  fft_4step<N1, N2, S::LINES, false, kTwByK, N1>(v, w, j, line, t, twm, false);
   float2* zrow = t + line * (M + 1);
   if (j < N2) {
 #pragma unroll
@@ -462,7 +467,7 @@ __global__ void __launch_bounds__((Shape4<N1, N2>::BLOCK)) k_irfft4s_rows(RowsAr
   __shared__ float2 twn[M + 1];
   __shared__ float2 t[t_size<N1, N2, S::LINES, false>()];
   const int tid = static_cast<int>(threadIdx.x);
-  for (int i = tid; i < M; i += S::BLOCK) twm[i] = a.tw[2 * i];
+  fill_twiddle_matrix<N2, N1, N1>(twm, a.tw, 2, tid, S::BLOCK);  // W_M^t = W_n^(2t)
   for (int i = tid; i <= M; i += S::BLOCK) twn[i] = a.tw[i];
   const int line = tid / S::T;
   const int j = tid % S::T;
@@ -506,7 +511,7 @@ __global__ void __launch_bounds__((Shape4<N1, N2>::BLOCK)) k_irfft4s_rows(RowsAr
     v[n2] = z;
   }
   __syncthreads();
-  fft_4step<N1, N2, S::LINES, false>(v, w, j, line, t, twm, true);
+  fft_4step<N1, N2, S::LINES, false, kTwByK, N1>(v, w, j, line, t, twm, true);
   if (valid && j < N2) {
 #pragma unroll
     for (int k1 = 0; k1 < N1; ++k1) {

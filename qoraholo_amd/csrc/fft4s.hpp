@@ -143,10 +143,35 @@ constexpr int t_size() {
   return LINE_FAST ? N1 * N2 * LINES : LINES * N1 * (N2 + 1);
 }
 
+// Twiddle layouts of fft_4step, where thread j needs W_N^(j k2) for k2 = 1 .. N2-1:
+//   kTwPlain: tw[t] = W_N^t (t < N), read at tw[j k2];
+//   kTwByK:   a matrix A[x][y] = W_N^(x y) at tw[x P + y] (fill_twiddle_matrix), read at
+//             A[k2][j] = tw[k2 P + j];
+//   kTwByJ:   the same matrix read by the transform with the factors swapped, A[j][k2].
+// Threads that walk along lines (j differs across the lanes of a wave) read tw[j k2] with a
+// stride of k2 entries, up to 16 lanes on one LDS bank (k2 = 16: 41% of the LDS cycles of the
+// 384-point BPM row kernels were bank conflicts); A[k2][j] is read at consecutive addresses,
+// and A[j][k2] with a stride of P = N1 + 1 entries (2 LDS banks per step) meets at most two
+// lanes per bank. One matrix with rows k2 < N2 and columns j < N1 serves both transforms of a
+// line engine. Threads that walk across lines share j within a line group and read kTwPlain.
+enum TwLayout { kTwPlain, kTwByK, kTwByJ };
+
+// A[x][y] = W_N^(x y) for x < R, y < C at dst[x P + y], gathered from src[stride t] = W_N^t
+// (the table's own values, so results do not change with the layout).
+template <int R, int C, int P>
+__device__ __forceinline__ void fill_twiddle_matrix(float2* dst, const float2* src, int stride,
+                                                    int tid, int threads) {
+  for (int i = tid; i < R * C; i += threads) {
+    const int x = i / C;
+    const int y = i - x * C;
+    dst[x * P + y] = src[stride * (x * y)];
+  }
+}
+
 // One line-batch transform. Thread j of `line` enters with v[n2] = x[j + N1 n2] (j < N1) and
-// leaves with w[k1] = X[N2 k1 + j] (j < N2). tw holds W_N^t, t < N (LDS). Every thread of the
-// block calls it; it ends with a barrier so `t` can be reused.
-template <int N1, int N2, int LINES, bool LINE_FAST>
+// leaves with w[k1] = X[N2 k1 + j] (j < N2); twiddles in LDS in layout TWL (pitch P). Every
+// thread of the block calls it; it ends with a barrier so `t` can be reused.
+template <int N1, int N2, int LINES, bool LINE_FAST, int TWL = kTwPlain, int P = 0>
 __device__ __forceinline__ void fft_4step(float2 (&v)[N2], float2 (&w)[N1], int j, int line,
                                           float2* t, const float2* tw, bool inv) {
   if (j < N1) {
@@ -154,7 +179,7 @@ __device__ __forceinline__ void fft_4step(float2 (&v)[N2], float2 (&w)[N1], int 
 #pragma unroll
     for (int k2 = 1; k2 < N2; ++k2) {
       if (j > 0) {
-        float2 c = tw[j * k2];
+        float2 c = TWL == kTwByK ? tw[k2 * P + j] : (TWL == kTwByJ ? tw[j * P + k2] : tw[j * k2]);
         if (inv) c.y = -c.y;
         v[k2] = cmul(v[k2], c);
       }

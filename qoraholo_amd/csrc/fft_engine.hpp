@@ -48,6 +48,12 @@ struct RegLines {
   static_assert(LINES >= 1 && BLOCK % T == 0, "a block holds whole lines");
   static constexpr int kLds =
       ce_max(t_size<N1, N2, LINES, LINE_FAST>(), t_size<N2, N1, LINES, LINE_FAST>());
+  // Threads that walk along lines read the twiddles from one matrix W^(x y), x < N2, y < N1,
+  // rows padded to N1 + 1 entries: by rows in the first transform, by columns in the second
+  // (fft_4step's kTwByK and kTwByJ). Threads that walk across lines read the plain table.
+  static constexpr bool kMatrix = !LINE_FAST;
+  static constexpr int kTwPitch = N1 + 1;
+  static constexpr int kTw = kMatrix ? N2 * kTwPitch : N;
 
   float2* t;
   float2* tw;
@@ -63,7 +69,7 @@ struct RegLines {
     return s;
   }
   __device__ static float2* lds_tw() {
-    __shared__ float2 s[N];
+    __shared__ float2 s[kTw];
     return s;
   }
   __device__ static double* lds_red() {
@@ -78,7 +84,11 @@ struct RegLines {
     t = lds_t();
     tw = lds_tw();
     const int tid = static_cast<int>(threadIdx.x);
-    for (int i = tid; i < N; i += BLOCK) tw[i] = g.tw[i];
+    if constexpr (kMatrix) {
+      fill_twiddle_matrix<N2, N1, kTwPitch>(tw, g.tw, 1, tid, BLOCK);
+    } else {
+      for (int i = tid; i < N; i += BLOCK) tw[i] = g.tw[i];
+    }
     line = LINE_FAST ? tid % LINES : tid / T;
     j = LINE_FAST ? tid / LINES : tid % T;
     l = xcd_block() * LINES + line;
@@ -94,7 +104,8 @@ struct RegLines {
     __syncthreads();
   }
   __device__ void first(bool inverse) {
-    fft_4step<N1, N2, LINES, LINE_FAST>(v, w, j, line, t, tw, inverse);
+    fft_4step<N1, N2, LINES, LINE_FAST, kMatrix ? kTwByK : kTwPlain, kTwPitch>(v, w, j, line, t, tw,
+                                                                             inverse);
   }
   template <class F>
   __device__ void mid(F f) {
@@ -121,7 +132,8 @@ struct RegLines {
     __syncthreads();
   }
   __device__ void second(bool inverse) {
-    fft_4step<N2, N1, LINES, LINE_FAST>(w, v, j, line, t, tw, inverse);
+    fft_4step<N2, N1, LINES, LINE_FAST, kMatrix ? kTwByJ : kTwPlain, kTwPitch>(w, v, j, line, t, tw,
+                                                                             inverse);
   }
   template <class F>
   __device__ void store1(F f) {
