@@ -126,6 +126,20 @@ __device__ __forceinline__ int line_in_plane(long l, int n) {
   return static_cast<int>(static_cast<unsigned>(l) % static_cast<unsigned>(line_side<E>(n)));
 }
 
+// Bins whose operands the slice kernels load together before using any of them (mid_loaded):
+// 8 for lines of 384 and more (384/81 gradient 210 -> 191 ms; its column blocks hold one block
+// per CU either way), 4 for shorter lines, where 8 would take the 256-point column kernels past
+// 128 registers and one of their two blocks per CU (256/49 gradient 32.4 -> 30.7 ms with 4, 32.8
+// with 8). Same results bit for bit.
+template <class E>
+constexpr int mid_group() {
+  if constexpr (requires { E::N; }) {
+    return E::N >= 384 ? 8 : 4;
+  } else {
+    return 1;
+  }
+}
+
 // Rows (line l = b n + y, element i at l n + i): inverse x transform of T, envelope, screen,
 // forward x transform. in == null: the unit envelope of the first slice. out == null: store the
 // envelope only.
@@ -145,11 +159,13 @@ __global__ void __launch_bounds__(E::kBlock) k_bpm_rows_fwd(Args a) {
   }
   e.load([&](long l, int i) { return a.in[l * n + i]; });
   e.first(true);
-  e.mid([&](long l, int k, float2& v) {
-    const float2 w = cscale(v, a.inv);
-    if (a.w) a.w[l * n + k] = w;
-    v = cmul(a.screen[line_in_plane<E>(l, n) * n + k], w);
-  });
+  e.template mid_loaded<mid_group<E>()>(
+      [&](long l, int k) { return a.screen[line_in_plane<E>(l, n) * n + k]; },
+      [&](long l, int k, float2& v, float2 t) {
+        const float2 w = cscale(v, a.inv);
+        if (a.w) a.w[l * n + k] = w;
+        v = cmul(t, w);
+      });
   if (a.out == nullptr) return;
   e.second(false);
   e.store2([&](long l, int m, float2 v) { a.out[l * n + m] = v; });
@@ -201,21 +217,32 @@ __global__ void __launch_bounds__(E::kBlock) k_bpm_cols_fwd(Args a) {
   const long plane = static_cast<long>(n) * n;
   e.load([&](long l, int i) { return a.in[col_at<E>(l, i, n, plane)]; });
   e.first(false);
-  e.mid([&](long l, int ky, float2& v) {
-    const long x = col_at<E>(l, ky, n, plane);
-    float2 s;
-    if (a.s_in) {
-      s = a.s_in[x];
-    } else if (a.s_zero) {
-      s = float2{0.0f, 0.0f};
-    } else {
-      s = (ky == 0 && line_in_plane<E>(l, n) == 0) ? float2{static_cast<float>(plane), 0.0f} : float2{0.0f, 0.0f};
-    }
-    const float2 c = a.c[x];
-    bpm_step(c.x, c.y, a.q[x], v.x, v.y, s.x, s.y);
-    a.s_out[x] = s;
-    v = s;
-  });
+  // Operands of a bin: the state S in front of the slice, the propagator c and the weight q.
+  struct Ops {
+    float2 s, c;
+    float q;
+  };
+  e.template mid_loaded<mid_group<E>()>(
+      [&](long l, int ky) {
+        const long x = col_at<E>(l, ky, n, plane);
+        Ops o;
+        if (a.s_in) {
+          o.s = a.s_in[x];
+        } else if (a.s_zero) {
+          o.s = float2{0.0f, 0.0f};
+        } else {
+          o.s = (ky == 0 && line_in_plane<E>(l, n) == 0) ? float2{static_cast<float>(plane), 0.0f}
+                                                          : float2{0.0f, 0.0f};
+        }
+        o.c = a.c[x];
+        o.q = a.q[x];
+        return o;
+      },
+      [&](long l, int ky, float2& v, Ops o) {
+        bpm_step(o.c.x, o.c.y, o.q, v.x, v.y, o.s.x, o.s.y);
+        a.s_out[col_at<E>(l, ky, n, plane)] = o.s;
+        v = o.s;
+      });
   if (a.out == nullptr) return;
   e.second(true);
   e.store2([&](long l, int m, float2 v) { a.out[col_at<E>(l, m, n, plane)] = v; });
@@ -260,22 +287,33 @@ __global__ void __launch_bounds__(E::kBlock) k_bpm_cols_adj(Args a) {
   const long plane = static_cast<long>(n) * n;
   e.load([&](long l, int i) { return a.in[col_at<E>(l, i, n, plane)]; });
   e.first(false);
-  e.mid([&](long l, int ky, float2& v) {
-    const long x = col_at<E>(l, ky, n, plane);
-    float gre = v.x;
-    float gim = v.y;
-    if (!a.first) {
-      const float2 g = a.g_state[x];
-      gre = g.x + v.x;
-      gim = g.y + v.y;
-    }
-    const float2 c = a.c[x];
-    float are = 0.0f;
-    float aim = 0.0f;
-    bpm_step_adjoint(c.x, c.y, a.q[x], gre, gim, are, aim);
-    a.g_state[x] = float2{gre, gim};
-    v = float2{are, aim};
-  });
+  // Operands of a bin: the carried adjoint state g, the propagator c and the weight q.
+  struct Ops {
+    float2 g, c;
+    float q;
+  };
+  e.template mid_loaded<mid_group<E>()>(
+      [&](long l, int ky) {
+        const long x = col_at<E>(l, ky, n, plane);
+        Ops o;
+        o.g = a.first ? float2{0.0f, 0.0f} : a.g_state[x];
+        o.c = a.c[x];
+        o.q = a.q[x];
+        return o;
+      },
+      [&](long l, int ky, float2& v, Ops o) {
+        float gre = v.x;
+        float gim = v.y;
+        if (!a.first) {
+          gre = o.g.x + v.x;
+          gim = o.g.y + v.y;
+        }
+        float are = 0.0f;
+        float aim = 0.0f;
+        bpm_step_adjoint(o.c.x, o.c.y, o.q, gre, gim, are, aim);
+        a.g_state[col_at<E>(l, ky, n, plane)] = float2{gre, gim};
+        v = float2{are, aim};
+      });
   e.second(true);
   e.store2([&](long l, int m, float2 v) { a.out[col_at<E>(l, m, n, plane)] = v; });
 }
@@ -290,13 +328,20 @@ k_bpm_rows_adj(Args a) {
   const int n = a.g.n;
   e.load([&](long l, int i) { return a.in[l * n + i]; });
   e.first(true);
-  e.mid([&](long l, int k, float2& v) {
-    const float2 b = cscale(v, a.inv);
-    const float2 t = a.screen[line_in_plane<E>(l, n) * n + k];
-    const float2 tw = cmul(t, a.w[l * n + k]);
-    a.p[l * n + k] = a.phase * (tw.x * b.y - tw.y * b.x);
-    v = cmul(cconj(t), b);
-  });
+  // Operands of a bin: the screen t and the stored envelope w.
+  struct Ops {
+    float2 t, w;
+  };
+  e.template mid_loaded<mid_group<E>()>(
+      [&](long l, int k) {
+        return Ops{a.screen[line_in_plane<E>(l, n) * n + k], a.w[l * n + k]};
+      },
+      [&](long l, int k, float2& v, Ops o) {
+        const float2 b = cscale(v, a.inv);
+        const float2 tw = cmul(o.t, o.w);
+        a.p[l * n + k] = a.phase * (tw.x * b.y - tw.y * b.x);
+        v = cmul(cconj(o.t), b);
+      });
   if (a.out == nullptr) return;
   e.second(false);
   e.store2([&](long l, int m, float2 v) { a.out[l * n + m] = v; });

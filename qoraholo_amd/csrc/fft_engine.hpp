@@ -5,6 +5,11 @@
 //   load(f)          input sample i of global line l is f(l, i)
 //   first(inverse)   transform every line
 //   mid(f)           f(l, k, v&) may read and replace output bin k of line l
+//   mid_loaded<G>(ld, f)  the same with operands: ld(l, k) loads bin k's operands, f(l, k, v&,
+//                    ops) uses them; the operands of G bins are loaded before any f runs.
+//                    f's stores may alias ld's loads as far as the compiler can tell, so in a
+//                    plain per-bin loop each bin waits for its own loads (measured: 16 to 32
+//                    memory round trips in a row per thread in the BPM slice kernels).
 //   reduce_lines(f, out)  out[l] = sum over k of f(l, k, v), summed in a fixed order
 //   second(inverse)  transform every line again
 //   store1(f) / store2(f)  f(l, m, v) receives element m after the first / second transform
@@ -112,6 +117,23 @@ struct RegLines {
     if (valid && j < N2) {
 #pragma unroll
       for (int k1 = 0; k1 < N1; ++k1) f(l, N2 * k1 + j, w[k1]);
+    }
+  }
+  template <int G, class L, class F>
+  __device__ void mid_loaded(L ld, F f) {
+    if (valid && j < N2) {
+#pragma unroll
+      for (int k0 = 0; k0 < N1; k0 += G) {
+        decltype(ld(l, 0)) ops[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          if (k0 + g < N1) ops[g] = ld(l, N2 * (k0 + g) + j);
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          if (k0 + g < N1) f(l, N2 * (k0 + g) + j, w[k0 + g], ops[g]);
+        }
+      }
     }
   }
   template <class F>
@@ -222,6 +244,10 @@ struct LdsLines {
       fn(l0 + line, k, cur[line * ls + k * es]);
     }
     __syncthreads();
+  }
+  template <int G, class L, class F>
+  __device__ void mid_loaded(L ld, F fn) {
+    mid([&](long l, int k, float2& v) { fn(l, k, v, ld(l, k)); });
   }
   template <class F>
   __device__ void reduce_lines(F fn, double* out) {
