@@ -42,8 +42,9 @@ def log(msg: str) -> None:
 
 
 def fgp_traffic(kernel: str):
-    """HBM bytes per launch of the FGP kernel measured by rocprofv3 (FETCH_SIZE + WRITE_SIZE,
-    scaled by the copy-kernel calibration), from the committed summary when it exists."""
+    """HBM bytes of the average FGP launch of a prox call measured by rocprofv3 (FETCH_SIZE +
+    WRITE_SIZE of each pass, FETCH_SIZE scaled by the copy-kernel calibration, averaged over the
+    passes), from the committed summary when it exists."""
     path = ROOT / "profiles" / f"{kernel}_256_hbm_traffic.txt"
     if not path.exists():
         return None
@@ -847,8 +848,9 @@ def main() -> int:
         "reference_model_frac": round(ref_bytes / (fgp_avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
     }
     if roofline["traffic"]:
-        # The physical rate: measured HBM bytes of a steady-state launch (13 volumes) over the
-        # mean launch duration.
+        # The physical rate: measured HBM bytes of the average launch of a prox call (the mean
+        # over its passes, which move 10, 13, 13, 13 and 10 volumes) over the mean launch
+        # duration of the same passes.
         roofline["traffic_gbs"] = round(roofline["traffic"] / (fgp_avg_ms * 1e-3) / 1e9, 1)
         roofline["traffic_frac"] = round(roofline["traffic_gbs"] / HBM_PEAK_GBS, 4)
     roofline["streaming_reference"] = streaming_reference(torch)
