@@ -48,7 +48,7 @@ __device__ __forceinline__ int xcd_tile() {
 // row or all outside it and every access is whole float4s (float2s when VX is not a multiple of
 // 4) under one predicate per lane; otherwise element by element.
 // XNEG: xb may be negative (the left halo of an x tile); other callers start rows at column 0.
-template <int VX, bool VEC, bool XNEG = false>
+template <int VX, bool VEC, bool XNEG = false, bool NT = false>
 __device__ __forceinline__ void load_row(const float* base, const TvArgs& a, int z, int y, int xb,
                                          float (&out)[VX]) {
   const bool inside = z >= 0 && z < a.nz && y >= 0 && y < a.ny;
@@ -59,7 +59,9 @@ __device__ __forceinline__ void load_row(const float* base, const TvArgs& a, int
       if constexpr (VX % 4 == 0) {
 #pragma unroll
         for (int v = 0; v < VX; v += 4) {
-          const float4 q = *reinterpret_cast<const float4*>(p + v);
+          typedef float vec4 __attribute__((ext_vector_type(4)));
+          const vec4 q = NT ? __builtin_nontemporal_load(reinterpret_cast<const vec4*>(p + v))
+                            : *reinterpret_cast<const vec4*>(p + v);
           out[v] = q.x;
           out[v + 1] = q.y;
           out[v + 2] = q.z;
@@ -68,7 +70,9 @@ __device__ __forceinline__ void load_row(const float* base, const TvArgs& a, int
       } else {
 #pragma unroll
         for (int v = 0; v < VX; v += 2) {
-          const float2 q = *reinterpret_cast<const float2*>(p + v);
+          typedef float vec2 __attribute__((ext_vector_type(2)));
+          const vec2 q = NT ? __builtin_nontemporal_load(reinterpret_cast<const vec2*>(p + v))
+                            : *reinterpret_cast<const vec2*>(p + v);
           out[v] = q.x;
           out[v + 1] = q.y;
         }
@@ -513,18 +517,32 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
   const bool out_row = w >= TOP && w < TOP + T && y < a.ny;
   const int pre = 2;  // planes of warm-up before z0
   // p_k, p_{k-1} and b of this row on plane f, as loaded.
-  auto load_raw = [&](int f, float (&Pk)[3][VX], float (&Pm)[3][VX], float (&B)[VX]) {
+  auto load_raw_as = [&]<bool NT>(int f, float (&Pk)[3][VX], float (&Pm)[3][VX], float (&B)[VX]) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      load_row<VX, VEC, XT>(a.p_k + c * a.vol, bd, f, y, xb, Pk[c]);
+      load_row<VX, VEC, XT, NT>(a.p_k + c * a.vol, bd, f, y, xb, Pk[c]);
       if (!a.first) {
-        load_row<VX, VEC, XT>(a.p_km1 + c * a.vol, bd, f, y, xb, Pm[c]);
+        load_row<VX, VEC, XT, NT>(a.p_km1 + c * a.vol, bd, f, y, xb, Pm[c]);
       } else {
 #pragma unroll
         for (int v = 0; v < VX; ++v) Pm[c][v] = 0.0f;
       }
     }
-    load_row<VX, VEC, XT>(a.b, bd, f, y, xb, B);
+    load_row<VX, VEC, XT, NT>(a.b, bd, f, y, xb, B);
+  };
+  // Rows that no other tile reads (waves 4 .. NW-5 of an untiled row: the halo rows of the
+  // tiles above and below are this tile's first and last two output rows) are loaded
+  // non-temporally, which keeps them from displacing the rows that neighbouring tiles read back
+  // (256^3 pass 153.6 -> 147.8 us). In x tiles every row's edge columns are a neighbouring
+  // tile's halo, and the same hint made the 384^3 pass slower (587 -> 608 us); so did loading
+  // every row non-temporally (256^3: 165 us).
+  const bool stream_row = !XT && w >= 4 && w <= NW - 5;
+  auto load_raw = [&](int f, float (&Pk)[3][VX], float (&Pm)[3][VX], float (&B)[VX]) {
+    if (stream_row) {
+      load_raw_as.template operator()<true>(f, Pk, Pm, B);
+    } else {
+      load_raw_as.template operator()<false>(f, Pk, Pm, B);
+    }
   };
   // r_k = p_k + m1 (p_k - p_{k-1}), or p_k on the first step.
   auto extrapolate_k = [&](const float (&Pk)[3][VX], const float (&Pm)[3][VX], float (&R)[3][VX]) {
