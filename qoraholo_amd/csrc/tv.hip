@@ -712,9 +712,16 @@ __global__ void __launch_bounds__(64) k_tv_primal(TvArgs a, int chunks, float* o
   for (int z = z0; z < z1; ++z) {
 #pragma unroll
     for (int j = 0; j < TY; ++j) {
-#pragma unroll
-      for (int c = 0; c < 3; ++c) load_row<VX, VEC>(a.p_in + c * a.vol, a, z, y0 + j, xb, p[c][j]);
-      load_row<VX, VEC>(a.b, a, z, y0 + j, xb, bb[j]);
+      // Read once (non-temporal), except the y component of the strip's last row, which the
+      // strip below reads as its row above (256^3 FISTA-TV request 20.33 -> 20.17 ms).
+      load_row<VX, VEC, false, true>(a.p_in, a, z, y0 + j, xb, p[0][j]);
+      if (j == TY - 1) {
+        load_row<VX, VEC>(a.p_in + a.vol, a, z, y0 + j, xb, p[1][j]);
+      } else {
+        load_row<VX, VEC, false, true>(a.p_in + a.vol, a, z, y0 + j, xb, p[1][j]);
+      }
+      load_row<VX, VEC, false, true>(a.p_in + 2 * a.vol, a, z, y0 + j, xb, p[2][j]);
+      load_row<VX, VEC, false, true>(a.b, a, z, y0 + j, xb, bb[j]);
     }
 #pragma unroll
     for (int v = 0; v < VX; ++v) {
