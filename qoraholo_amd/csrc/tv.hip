@@ -300,12 +300,11 @@ __global__ void __launch_bounds__(64) k_fgp_step(TvArgs a, int chunks) {
 // side make step one available where step two needs it; their results are not stored. Every
 // value is formed with the float operations of k_fgp_step, so a pass equals two single steps
 // bit for bit, while HBM sees 7 volumes read and 6 written per two steps instead of 20.
-// Planes per block: 64, halved (down to 8) while that leaves the grid short of one block per
-// CU (256^3: 32 planes, 256 blocks; 384^3: 64 planes, 768 blocks); each chunk starts two planes
-// early, so longer chunks read fewer warm-up planes (384^3 pass 675 -> 637 us from 32 to 64).
+// Planes per block: at most 64 and at least 4, chosen per volume by pair_chunk; each chunk
+// starts two planes early, so longer chunks read fewer warm-up planes (384^3 pass 675 -> 637 us
+// from 32 to 64 planes).
 constexpr int kPairChunk = 64;
-constexpr int kPairChunkMin = 8;
-constexpr int kPairTargetBlocks = 256;
+constexpr int kPairChunkMin = 4;
 
 struct PairArgs {
   const float* b;
@@ -854,17 +853,54 @@ int pair_waves(int vx) { return vx == 1 ? 16 : (vx == 2 ? 16 : 12); }
 constexpr int kTileVX = 2;
 constexpr int kPairTileX = 62 * kTileVX;
 
+// Planes per z chunk for a grid of `tiles` row tiles: the length from kPairChunkMin to
+// kPairChunk that minimises the rounds of blocks each CU runs times the planes a block marches
+// (its chunk and two warm-up planes), the longest on ties. 256^3: 32 planes, 256 blocks, one
+// round; 384^3: 64 planes, three rounds; 128^3: 6 planes and 242 blocks (8-plane chunks left 80
+// CUs idle and spent a fifth of each block on warm-up).
+int pair_chunk(const void* kernel, int threads, std::size_t lds, int tiles, int nz) {
+  int dev = 0;
+  check(hipGetDevice(&dev), "hipGetDevice");
+  int cus = 0;
+  check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev),
+        "hipDeviceGetAttribute");
+  int per_cu = 0;
+  check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, lds),
+        "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+  const long slots = static_cast<long>(std::max(cus, 1)) * std::max(per_cu, 1);
+  int best = kPairChunk;
+  long best_cost = -1;
+  for (int zl = kPairChunk; zl >= kPairChunkMin; --zl) {
+    const long blocks = static_cast<long>(tiles) * ((nz + zl - 1) / zl);
+    const long cost = (blocks + slots - 1) / slots * (zl + 2);
+    if (best_cost < 0 || cost < best_cost) {
+      best_cost = cost;
+      best = zl;
+    }
+  }
+  return best;
+}
+
+template <int VX, int NW, bool VEC, bool XT>
+void launch_pair_kernel(PairArgs a, int tiles, std::size_t lds, hipStream_t s) {
+  a.zlen = pair_chunk(reinterpret_cast<const void*>(&k_fgp_pair<VX, NW, VEC, XT>), NW * 64, lds,
+                      tiles, a.nz);
+  const int chunks = (a.nz + a.zlen - 1) / a.zlen;
+  hipLaunchKernelGGL((k_fgp_pair<VX, NW, VEC, XT>), dim3(static_cast<unsigned>(tiles * chunks)),
+                     dim3(NW * 64), lds, s, a, chunks);
+}
+
 template <int VX, int NW, bool XT>
-void launch_pair(const PairArgs& a, int chunks, dim3 grid, hipStream_t s) {
+void launch_pair(const PairArgs& a, int tiles, hipStream_t s) {
   // Four row buffers of NW rows of nx floats (64 VX per x tile): at most 48 KiB.
   const std::size_t lds = 4 * static_cast<std::size_t>(NW) * (XT ? 64 * VX : a.nx) * sizeof(float);
   if constexpr (VX % 2 == 0) {
     if (rows_vec<VX>(a.nx)) {
-      hipLaunchKernelGGL((k_fgp_pair<VX, NW, true, XT>), grid, dim3(NW * 64), lds, s, a, chunks);
+      launch_pair_kernel<VX, NW, true, XT>(a, tiles, lds, s);
       return;
     }
   }
-  hipLaunchKernelGGL((k_fgp_pair<VX, NW, false, XT>), grid, dim3(NW * 64), lds, s, a, chunks);
+  launch_pair_kernel<VX, NW, false, XT>(a, tiles, lds, s);
 }
 
 }  // namespace
@@ -886,21 +922,16 @@ void fgp_pair(const float* b, const float* p_k, const float* p_km1, float* p_k1,
   const int nw = pair_waves(vx);
   const int rows = nw - 4;
   const int tiles_y = (ny + rows - 1) / rows;
-  int zlen = kPairChunk;
-  while (zlen > kPairChunkMin && tiles_x * tiles_y * ((nz + zlen - 1) / zlen) < kPairTargetBlocks) {
-    zlen /= 2;
-  }
+  const int tiles = tiles_x * tiles_y;
   PairArgs a{b,  p_k,    p_km1, p_k1, p_k2,          static_cast<long>(nz) * ny * nx,
-             nz, ny,     nx,    lambda, step,        m1, first ? 1 : 0, m2, zlen, xt, tiles_x};
-  const int chunks = (nz + zlen - 1) / zlen;
-  const dim3 grid(static_cast<unsigned>(tiles_x * tiles_y * chunks));
+             nz, ny,     nx,    lambda, step,        m1, first ? 1 : 0, m2, 0, xt, tiles_x};
   if (tiles_x > 1) {
-    launch_pair<kTileVX, kTileVX == 2 ? 16 : 12, true>(a, chunks, grid, s);
+    launch_pair<kTileVX, kTileVX == 2 ? 16 : 12, true>(a, tiles, s);
   } else {
     switch (vx) {
-      case 1: launch_pair<1, 16, false>(a, chunks, grid, s); break;
-      case 2: launch_pair<2, 16, false>(a, chunks, grid, s); break;
-      default: launch_pair<4, 12, false>(a, chunks, grid, s); break;
+      case 1: launch_pair<1, 16, false>(a, tiles, s); break;
+      case 2: launch_pair<2, 16, false>(a, tiles, s); break;
+      default: launch_pair<4, 12, false>(a, tiles, s); break;
     }
   }
   check_launch("fgp_pair");
