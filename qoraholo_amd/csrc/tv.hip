@@ -33,6 +33,7 @@ struct TvArgs {
   float step;
   float momentum;  // m_{k-1}, the momentum of the step that produced p_k
   int first;       // r = p_k exactly
+  int zlen = kZChunk;  // planes per chunk of the strip kernels (strip_grid)
 };
 
 // Block b runs tile t: with a grid that is a multiple of 8, blocks b, b+8, b+16, ... (which the
@@ -217,8 +218,8 @@ template <int VX, int TY, bool VEC>
 __global__ void __launch_bounds__(64) k_fgp_step(TvArgs a, int chunks) {
   const int t = xcd_tile();
   const int y0 = (t / chunks) * TY;
-  const int z0 = (t % chunks) * kZChunk;
-  const int z1 = min(z0 + kZChunk, a.nz);
+  const int z0 = (t % chunks) * a.zlen;
+  const int z1 = min(z0 + a.zlen, a.nz);
   const int xb = static_cast<int>(threadIdx.x) * VX;
 
   float r[3][TY + 1][VX];  // r at plane z, rows y0..y0+TY
@@ -696,8 +697,8 @@ __global__ void __launch_bounds__(64) k_tv_primal(TvArgs a, int chunks, float* o
                                                   float nm2, float inv_k02) {
   const int t = xcd_tile();
   const int y0 = (t / chunks) * TY;
-  const int z0 = (t % chunks) * kZChunk;
-  const int z1 = min(z0 + kZChunk, a.nz);
+  const int z0 = (t % chunks) * a.zlen;
+  const int z1 = min(z0 + a.zlen, a.nz);
   const int xb = static_cast<int>(threadIdx.x) * VX;
   float p[3][TY + 1][VX];
   float pz_below[TY + 1][VX];
@@ -769,9 +770,22 @@ int lanes_x(int nx) {
   throw InvalidArgument("tv: volume side above 512");
 }
 
-dim3 strip_grid(int ny, int nz, int* chunks) {
-  *chunks = (nz + kZChunk - 1) / kZChunk;
-  return dim3(static_cast<unsigned>(((ny + kRows - 1) / kRows) * *chunks));
+// One wave per strip of kRows rows and z chunk: chunks of kZChunk planes, halved (down to 2)
+// while the grid holds fewer than two waves per SIMD (128^3: 4 planes and 2048 waves, where
+// 16-plane chunks gave 512 waves for 1024 SIMDs).
+dim3 strip_grid(int ny, int nz, TvArgs& a, int* chunks) {
+  int dev = 0;
+  check(hipGetDevice(&dev), "hipGetDevice");
+  int cus = 0;
+  check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev),
+        "hipDeviceGetAttribute");
+  const long strips = (ny + kRows - 1) / kRows;
+  const long want = 8L * std::max(cus, 1);
+  int zlen = kZChunk;
+  while (zlen > 2 && strips * ((nz + zlen - 1) / zlen) < want) zlen /= 2;
+  a.zlen = zlen;
+  *chunks = (nz + zlen - 1) / zlen;
+  return dim3(static_cast<unsigned>(strips * *chunks));
 }
 
 // Whole-vector row access (see load_row) when the row splits into whole lane chunks.
@@ -813,7 +827,7 @@ void fgp_step(const float* b, const float* p_in, const float* p_prev, float* p_o
   TvArgs a{b, p_in, p_prev, p_out, static_cast<long>(nz) * ny * nx, nz, ny, nx,
            lambda, step, momentum, first ? 1 : 0};
   int chunks = 0;
-  const dim3 grid = strip_grid(ny, nz, &chunks);
+  const dim3 grid = strip_grid(ny, nz, a, &chunks);
   switch (lanes_x(nx)) {
     case 1: launch_step<1>(a, chunks, grid, s); break;
     case 2: launch_step<2>(a, chunks, grid, s); break;
@@ -829,7 +843,7 @@ void tv_primal(const float* b, const float* p, int nz, int ny, int nx, float lam
   TvArgs a{b, p, nullptr, nullptr, static_cast<long>(nz) * ny * nx, nz, ny, nx,
            lambda, 0.0f, 0.0f, 1};
   int chunks = 0;
-  const dim3 grid = strip_grid(ny, nz, &chunks);
+  const dim3 grid = strip_grid(ny, nz, a, &chunks);
   const int ti = to_index ? 1 : 0;
   switch (lanes_x(nx)) {
     case 1: launch_primal<1>(a, chunks, grid, out, ti, nm2, inv_k02, s); break;
