@@ -12,6 +12,9 @@
 // uses to produce it (src/common.hpp:128-141), so a step reads b, p_k, p_{k-1} and writes p_{k+1}
 // (10 volumes instead of 13 with r stored). The first step of a prox uses r = p_k
 // (src/cpu/tv.cpp:10).
+#include <algorithm>
+#include <atomic>
+
 #include "common.hpp"
 #include "runtime.hpp"
 #include "tv.hpp"
@@ -770,17 +773,28 @@ int lanes_x(int nx) {
   throw InvalidArgument("tv: volume side above 512");
 }
 
+// CUs of the current device, queried once per device.
+int device_cus() {
+  static std::atomic<int> cache[64];
+  int dev = 0;
+  check(hipGetDevice(&dev), "hipGetDevice");
+  require(dev >= 0 && dev < 64, "tv: device index above 63");
+  int cus = cache[dev].load(std::memory_order_relaxed);
+  if (cus == 0) {
+    check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev),
+          "hipDeviceGetAttribute");
+    cus = std::max(cus, 1);
+    cache[dev].store(cus, std::memory_order_relaxed);
+  }
+  return cus;
+}
+
 // One wave per strip of kRows rows and z chunk: chunks of kZChunk planes, halved (down to 2)
 // while the grid holds fewer than two waves per SIMD (128^3: 4 planes and 2048 waves, where
 // 16-plane chunks gave 512 waves for 1024 SIMDs).
 dim3 strip_grid(int ny, int nz, TvArgs& a, int* chunks) {
-  int dev = 0;
-  check(hipGetDevice(&dev), "hipGetDevice");
-  int cus = 0;
-  check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev),
-        "hipDeviceGetAttribute");
   const long strips = (ny + kRows - 1) / kRows;
-  const long want = 8L * std::max(cus, 1);
+  const long want = 8L * device_cus();
   int zlen = kZChunk;
   while (zlen > 2 && strips * ((nz + zlen - 1) / zlen) < want) zlen /= 2;
   a.zlen = zlen;
@@ -872,16 +886,16 @@ constexpr int kPairTileX = 62 * kTileVX;
 // (its chunk and two warm-up planes), the longest on ties. 256^3: 32 planes, 256 blocks, one
 // round; 384^3: 64 planes, three rounds; 128^3: 6 planes and 242 blocks (8-plane chunks left 80
 // CUs idle and spent a fifth of each block on warm-up).
-int pair_chunk(const void* kernel, int threads, std::size_t lds, int tiles, int nz) {
-  int dev = 0;
-  check(hipGetDevice(&dev), "hipGetDevice");
-  int cus = 0;
-  check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev),
-        "hipDeviceGetAttribute");
-  int per_cu = 0;
-  check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, lds),
-        "hipOccupancyMaxActiveBlocksPerMultiprocessor");
-  const long slots = static_cast<long>(std::max(cus, 1)) * std::max(per_cu, 1);
+int pair_chunk(const void* kernel, std::atomic<int>& per_cu_cache, int threads, std::size_t lds,
+               int tiles, int nz) {
+  int per_cu = per_cu_cache.load(std::memory_order_relaxed);
+  if (per_cu == 0) {
+    check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, lds),
+          "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+    per_cu = std::max(per_cu, 1);
+    per_cu_cache.store(per_cu, std::memory_order_relaxed);
+  }
+  const long slots = static_cast<long>(device_cus()) * per_cu;
   int best = kPairChunk;
   long best_cost = -1;
   for (int zl = kPairChunk; zl >= kPairChunkMin; --zl) {
@@ -897,8 +911,10 @@ int pair_chunk(const void* kernel, int threads, std::size_t lds, int tiles, int 
 
 template <int VX, int NW, bool VEC, bool XT>
 void launch_pair_kernel(PairArgs a, int tiles, std::size_t lds, hipStream_t s) {
-  a.zlen = pair_chunk(reinterpret_cast<const void*>(&k_fgp_pair<VX, NW, VEC, XT>), NW * 64, lds,
-                      tiles, a.nz);
+  // Blocks per CU of this kernel (its registers and LDS; the same on every gfx950 device).
+  static std::atomic<int> per_cu{0};
+  a.zlen = pair_chunk(reinterpret_cast<const void*>(&k_fgp_pair<VX, NW, VEC, XT>), per_cu,
+                      NW * 64, lds, tiles, a.nz);
   const int chunks = (a.nz + a.zlen - 1) / a.zlen;
   hipLaunchKernelGGL((k_fgp_pair<VX, NW, VEC, XT>), dim3(static_cast<unsigned>(tiles * chunks)),
                      dim3(NW * 64), lds, s, a, chunks);
