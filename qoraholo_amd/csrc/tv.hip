@@ -873,13 +873,15 @@ namespace {
 
 // Rows per block of the two-step kernel: NW waves, NW - 4 output rows, as many waves as the
 // registers of VX floats per lane leave room for in one block per CU. Rows up to 256 wide are
-// held whole by a wave; wider rows are cut into x tiles of at most 62 kTileVX output columns
-// (one lane of kTileVX columns of halo on either side), balanced. Tiles of 2 columns per lane
-// run 16 waves (12 output rows) and leave fewer lanes idle than tiles of 4 (384: 4 tiles of 96
-// columns, 52 of 64 lanes busy, against 2 tiles of 192, 50 lanes, and 8 output rows).
+// held whole by a wave; wider rows are cut into x tiles of at most 62 VX output columns (one
+// lane of VX columns of halo on either side), balanced. Tiles of 4 columns per lane (12 waves,
+// 8 output rows) when the row width is a multiple of 4, so that every access is a float4: at
+// 384^3, 2 tiles of 192 columns ran a pass in 593 us against 618 us for 4 tiles of 96 at 2
+// columns per lane (16 waves, 12 output rows), which leave fewer lanes and rows idle; at 512^3
+// and 320^3 the two were equal (1667 against 1658 us, 417 against 420 us). Other widths take
+// tiles of 2 columns per lane, whole float2s for even widths.
 int pair_waves(int vx) { return vx == 1 ? 16 : (vx == 2 ? 16 : 12); }
-constexpr int kTileVX = 2;
-constexpr int kPairTileX = 62 * kTileVX;
+int pair_tile_vx(int nx) { return nx % 4 == 0 ? 4 : 2; }
 
 // Planes per z chunk for a grid of `tiles` row tiles: the length from kPairChunkMin to
 // kPairChunk that minimises the rounds of blocks each CU runs times the planes a block marches
@@ -945,8 +947,9 @@ void fgp_pair(const float* b, const float* p_k, const float* p_km1, float* p_k1,
   int tiles_x = 1;
   int xt = nx;
   if (nx > 64 * 4) {
-    vx = kTileVX;
-    tiles_x = (nx + kPairTileX - 1) / kPairTileX;
+    vx = pair_tile_vx(nx);
+    const int widest = 62 * vx;
+    tiles_x = (nx + widest - 1) / widest;
     xt = ((nx + tiles_x - 1) / tiles_x + vx - 1) / vx * vx;  // balanced tiles, whole lanes
   }
   const int nw = pair_waves(vx);
@@ -956,7 +959,11 @@ void fgp_pair(const float* b, const float* p_k, const float* p_km1, float* p_k1,
   PairArgs a{b,  p_k,    p_km1, p_k1, p_k2,          static_cast<long>(nz) * ny * nx,
              nz, ny,     nx,    lambda, step,        m1, first ? 1 : 0, m2, 0, xt, tiles_x};
   if (tiles_x > 1) {
-    launch_pair<kTileVX, kTileVX == 2 ? 16 : 12, true>(a, tiles, s);
+    if (vx == 4) {
+      launch_pair<4, 12, true>(a, tiles, s);
+    } else {
+      launch_pair<2, 16, true>(a, tiles, s);
+    }
   } else {
     switch (vx) {
       case 1: launch_pair<1, 16, false>(a, tiles, s); break;
