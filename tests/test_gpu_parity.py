@@ -195,13 +195,15 @@ def test_tv_prox_partial_tiles(device, oracle):
                                          ((20, 30, 300), 3), ((44, 50, 128), 6),
                                          ((96, 100, 384), 4), ((12, 18, 302), 5),
                                          ((10, 12, 400), 3), ((24, 30, 260), 3),
-                                         ((6, 20, 512), 4), ((8, 9, 505), 3)])
+                                         ((6, 20, 512), 4), ((8, 9, 505), 3),
+                                         ((384, 384, 384), 4), ((128, 128, 128), 5)])
 def test_tv_two_step_passes_equal_single_steps(device, shape, inner):
     """Two FGP steps per pass give the bits of one step per pass: partial row tiles and z
     chunks, odd inner counts (a single last step), row widths for 1, 2 and 4 floats per lane
     with and without whole-vector access, the full 256^3 volume, a small volume (short z
-    chunks), and rows wider than 256 cut into two or three x tiles (384, 302, 400, 260, 512
-    and 505 wide, the last with element-wise access)."""
+    chunks), rows wider than 256 cut into x tiles of 4 columns per lane (384, 400, 260, 512)
+    or 2 (302, and 505 with element-wise access), the full 384^3 volume (64-plane chunks) and
+    128^3 (6-plane chunks)."""
     from qoraholo_amd import _native
 
     rng = np.random.default_rng(shape[0])
