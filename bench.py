@@ -888,6 +888,12 @@ def main() -> int:
         roofline["traffic_gbs"] = round(roofline["traffic"] / (fgp_avg_ms * 1e-3) / 1e9, 1)
         roofline["traffic_frac"] = round(roofline["traffic_gbs"] / HBM_PEAK_GBS, 4)
     roofline["streaming_reference"] = streaming_reference(torch)
+    if roofline.get("traffic_gbs"):
+        # The measured traffic rate against what plain streaming kernels reach on this box in
+        # this run (the peak in `frac` is the datasheet's).
+        ref = roofline["streaming_reference"]
+        roofline["traffic_vs_copy"] = round(roofline["traffic_gbs"] / ref["copy_gbs"], 4)
+        roofline["traffic_vs_addcmul"] = round(roofline["traffic_gbs"] / ref["addcmul_gbs"], 4)
     recon_bytes = reconstruction_bytes(ANGLES, 2 * N, N, N, ITERATIONS, INNER)
     per_recon = {
         "bytes": recon_bytes,
