@@ -259,9 +259,10 @@ BPM_ANGLES = 49
 BPM_GEOM = (0.532, 1.337, 1.2, 0.2)
 
 
-def bpm_bytes(n: int, nz: int, count: int, checkpoint: int = 16) -> tuple[int, int]:
+def bpm_bytes(n: int, nz: int, count: int, checkpoint: int) -> tuple[int, int]:
     """Minimum bytes of the fused BPM kernels (forward model; data-term gradient with the
-    adjoint recomputing envelopes from checkpoints every `checkpoint` slices), per pixel per
+    adjoint recomputing envelopes from checkpoints every `checkpoint` slices, or with every
+    envelope stored by the forward when `checkpoint` is 0), per pixel per
     illumination per slice: forward rows 16 (T in, U out) and columns 44 (U, S, step, q in;
     S, T out) = 60; a recomputed slice also stores its envelope (+8); the adjoint columns 44,
     rows 28 (in, envelope, out, gradient term) and the ordered reduction 4. Per slice and not
@@ -325,7 +326,7 @@ def bpm_leg(qh, torch, cpu=True, repeats=5):
     log(f"bpm: one inversion iteration {iter_ms:.2f} ms (host arrays, from 1 and 6 iterations)")
 
     a, plane, nz = BPM_ANGLES, BPM_N * BPM_N, BPM_N
-    fwd_alg, grad_alg = bpm_bytes(BPM_N, BPM_N, a)
+    fwd_alg, grad_alg = bpm_bytes(BPM_N, BPM_N, a, plan.checkpoint)
     # The reference's traffic model (88 bytes per pixel per slice per illumination for the
     # forward, two cuFFT transforms and two elementwise passes per slice; 222 GB for a gradient
     # with recomputation from checkpoints every 16 slices).
@@ -341,7 +342,8 @@ def bpm_leg(qh, torch, cpu=True, repeats=5):
 
     out = {
         "workload": "3 overlapping beads, 256^2 x 256 at 0.2 um, 49 illuminations NA 0.9, "
-                    "checkpoint every 16 slices",
+                    + (f"checkpoint every {plan.checkpoint} slices" if plan.checkpoint
+                       else "every envelope stored"),
         "forward_ms": round(fwd_ms, 3),
         "gradient_ms": round(grad_ms, 3),
         "iteration_ms_host_arrays": round(iter_ms, 3),
@@ -478,8 +480,9 @@ def served_bpm(qh, sizes=((192, 49), (384, 81)), iterations=40, seed=0, repeats=
     then 40 full-batch FISTA iterations of the BPM with the TV prox (weight 1e-2, 10 inner
     steps). Sizes 192^2 x 192 with 49 illuminations and 384^2 x 384 with 81; random cells at
     0.2 um. First each size through the per-call functions, one at a time (the stage split and
-    the per-iteration time; per iteration, the reference's traffic model of a gradient --
-    forward, 15/16 recomputation from checkpoints, adjoint -- is the algorithmic figure); then
+    the per-iteration time; per iteration, the minimum bytes of a gradient at the checkpoint
+    interval the model resolves to are the algorithmic figure, beside the reference's traffic
+    model -- forward, 15/16 recomputation from checkpoints, adjoint); then
     the same requests through the Server, its BPM geometries resident and admitted by their
     measured HBM footprint: a burst of `repeats[n]` requests of side n (4 of 192, 2 of 384),
     interleaved, with inversions/s, p50/p99 latency and each result compared with the per-call
@@ -509,7 +512,10 @@ def served_bpm(qh, sizes=((192, 49), (384, 81)), iterations=40, seed=0, repeats=
         qh.bpm_reconstruct(f, il, g, start, iterations=2, tv_weight=TV_WEIGHT, tv_inner=INNER)
         s4 = time.perf_counter()
         per_iter = 1000.0 * ((s3 - s2) - (s4 - s3)) / (iterations - 2)
-        grad_alg = bpm_bytes(n, n, a)[1]
+        # The per-call functions resolve the checkpoint interval from the free memory as a
+        # plan built now does (0 when every envelope of the illuminations fits).
+        checkpoint = qh.BpmPlan(n, n, il, g).checkpoint
+        grad_alg = bpm_bytes(n, n, a, checkpoint)[1]
         grad_ref = int((88 * (1 + 15 / 16) + 100) * n * n * a * n)
         truth = req.truth.astype(np.float64)
         row = {
@@ -522,6 +528,7 @@ def served_bpm(qh, sizes=((192, 49), (384, 81)), iterations=40, seed=0, repeats=
             "gp_start_ms": round(1000.0 * (s2 - s1), 1),
             "bpm_ms": round(1000.0 * (s3 - s2), 1),
             "iteration_ms": round(per_iter, 2),
+            "checkpoint": checkpoint,
             "iteration_roofline": {"bound": "hbm", "algorithmic_bytes": grad_alg,
                                    "achieved": round(grad_alg / (per_iter * 1e-3) / 1e9, 1),
                                    "peak": HBM_PEAK_GBS, "unit": "GB/s",

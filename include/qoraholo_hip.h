@@ -304,7 +304,9 @@ typedef struct {
   int obliquity;  /* weight the scattered part of each slice by its obliquity (reference: 1) */
   int chunk;      /* illuminations per device pass; 0: as many as keep the adjoint's buffers
                    * within half the free device memory */
-  int checkpoint; /* slices per recomputed segment of the adjoint; 0: keep every envelope */
+  int checkpoint; /* slices per recomputed segment of the adjoint; 0: keep every envelope;
+                   * -1: keep every envelope when one pass's illuminations then fit within half
+                   * the free device memory (what the reference's CUDA backend stores), else 16 */
 } qh_bpm_options;
 
 /* Reference BpmSolveOptions (include/qoraholo/bpm.hpp:32-45). */
@@ -408,6 +410,9 @@ void qh_hip_bpm_plan_destroy(qh_bpm_plan* plan);
 /* Screen phase, Lipschitz constant, illuminations per pass and device bytes of the plan. */
 int qh_hip_bpm_plan_info(const qh_bpm_plan* plan, float* phase, float* lipschitz, int* chunk,
                          size_t* device_bytes);
+/* The plan's checkpoint interval as resolved (0: every envelope kept); no counterpart in the
+ * reference, whose CUDA backend always keeps every envelope (src/cuda/bpm.cu:139-141). */
+int qh_hip_bpm_plan_checkpoint(const qh_bpm_plan* plan, int* checkpoint);
 int qh_hip_bpm_forward_dev(qh_bpm_plan* plan, const float* x, qh_c64* out, void* stream);
 int qh_hip_bpm_vjp_dev(qh_bpm_plan* plan, const float* x, const qh_c64* r, float* grad,
                        void* stream);

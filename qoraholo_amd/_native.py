@@ -166,6 +166,7 @@ _SIGNATURES = {
     "qh_hip_bpm_plan_destroy": (None, [c_void_p]),
     "qh_hip_bpm_plan_info": (c_int, [c_void_p, POINTER(c_float), POINTER(c_float), POINTER(c_int),
                                      POINTER(c_size_t)]),
+    "qh_hip_bpm_plan_checkpoint": (c_int, [c_void_p, POINTER(c_int)]),
     "qh_hip_bpm_forward_dev": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "qh_hip_bpm_vjp_dev": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "qh_hip_bpm_jvp_dev": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),

@@ -10,7 +10,10 @@ reference's src/api.cpp:159-163 does; results get n_medium added back in float32
 
 Device options beyond the reference: ``chunk`` (illuminations per device pass; 0 picks the
 largest that keeps the adjoint's buffers within half the free device memory) and
-``checkpoint`` (slices per recomputed segment of the adjoint; 0 keeps every envelope).
+``checkpoint`` (slices per recomputed segment of the adjoint; 0 keeps every envelope, as the
+reference's CUDA backend does, src/cuda/bpm.cu:139-141; -1, the default, keeps every envelope
+when one pass's illuminations then fit within half the free device memory and otherwise
+recomputes them from checkpoints every 16 slices).
 """
 
 from __future__ import annotations
@@ -21,7 +24,7 @@ import numpy as np
 
 from . import _native
 
-DEFAULT_CHECKPOINT = 16
+DEFAULT_CHECKPOINT = -1
 
 
 def _backend(backend: str) -> None:
@@ -287,6 +290,9 @@ class BpmPlan:
                                                          ctypes.byref(ch), ctypes.byref(nbytes)))
         self.phase, self.lipschitz, self.chunk = phase.value, lip.value, ch.value
         self._bytes = nbytes
+        cp = ctypes.c_int()
+        _native.check(_native.lib().qh_hip_bpm_plan_checkpoint(h, ctypes.byref(cp)))
+        self.checkpoint = cp.value  # as resolved: 0 when every envelope is kept
 
     def __del__(self):
         # The destroy function is held by the object: module globals may already be gone when

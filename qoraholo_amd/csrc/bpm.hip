@@ -489,8 +489,21 @@ BpmModel::BpmModel(const BpmTables& t, int chunk, int checkpoint)
     : n_(t.n), nz_(t.nz), count_(t.count), plane_(static_cast<long>(t.n) * t.n),
       phase_(t.phase), lipschitz_(t.lipschitz) {
   require(fft_supported(n_), "bpm: n must factor into 2, 3 and 5 and be at most 2048");
-  require(chunk >= 0 && checkpoint >= 0, "bpm: chunk and checkpoint must be non-negative");
+  require(chunk >= 0 && checkpoint >= -1, "bpm: chunk must be non-negative, checkpoint at least -1");
   require(static_cast<long>(t.count) * n_ < (1L << 31), "bpm: too many lines for one plan");
+  if (checkpoint < 0) {
+    // Every envelope kept when the illuminations of one pass (the chunk asked for, or all of
+    // them) then fit within half the free device memory: the adjoint recomputes nothing, at
+    // 8 n^2 nz bytes per illumination (384^2 x 384 with 81 illuminations: 37 GB, gradient
+    // 184 -> about 130 ms against checkpoints every 16 slices). Otherwise checkpoints keep the
+    // passes wide.
+    std::size_t free_bytes = 0, total_bytes = 0;
+    check(hipMemGetInfo(&free_bytes, &total_bytes), "hipMemGetInfo");
+    const std::size_t keep_all = static_cast<std::size_t>(plane_) *
+                                 (5 * sizeof(float2) + sizeof(float) + sizeof(float2) * nz_);
+    const std::size_t angles = static_cast<std::size_t>(chunk > 0 ? std::min(chunk, count_) : count_);
+    checkpoint = keep_all * angles <= free_bytes / 2 ? 0 : kBpmCheckpoint;
+  }
   seg_len_ = checkpoint == 0 || checkpoint > nz_ ? nz_ : checkpoint;
   nseg_ = (nz_ + seg_len_ - 1) / seg_len_;
   if (chunk == 0) {

@@ -34,7 +34,11 @@ BpmTables bpm_tables(int n, int nz, const qh_illumination* illuminations, int co
 // Illuminations are processed in chunks of `chunk` (0: as many as keep the adjoint's buffers
 // within half the free device memory); the adjoint recomputes envelopes from a checkpoint of
 // the propagation state every `checkpoint` slices (0: keep the envelope of every slice, no
-// recomputation). The adjoint's buffers are allocated by the first gradient or vjp.
+// recomputation; negative: keep every envelope when the chunk of illuminations asked for, or
+// all of them, then fits within half the free device memory, as src/cuda/bpm.cu:139-141 stores
+// them all, else checkpoint every kBpmCheckpoint slices). The adjoint's buffers are allocated
+// by the first gradient or vjp.
+constexpr int kBpmCheckpoint = 16;
 class BpmModel {
  public:
   BpmModel(const BpmTables& tables, int chunk, int checkpoint);
@@ -45,6 +49,8 @@ class BpmModel {
   int nz() const { return nz_; }
   int count() const { return count_; }
   int chunk() const { return chunk_; }
+  // Slices per recomputed segment as resolved (0: every envelope kept).
+  int checkpoint() const { return seg_len_ == nz_ ? 0 : seg_len_; }
   float phase() const { return phase_; }
   float lipschitz() const { return lipschitz_; }
   std::size_t bytes() const;

@@ -19,7 +19,7 @@ namespace qh {
 struct BpmInversionSpec {
   BpmTables tables;
   int chunk = 0;        // illuminations per device pass (0: chosen from the free memory)
-  int checkpoint = 16;  // slices per recomputed adjoint segment
+  int checkpoint = -1;  // slices per recomputed adjoint segment (-1: chosen from the free memory)
   OdtGrid grid;         // the start's reconstruction grid (n, nz of the BPM volume)
   OdtConfig start;      // Gerchberg-Papoulis start
   std::vector<float> illuminations;  // (count, 2): fy, fx
@@ -41,6 +41,7 @@ class BpmInversion {
   int nz() const { return model_.nz(); }
   int count() const { return model_.count(); }
   int chunk() const { return model_.chunk(); }
+  int checkpoint() const { return model_.checkpoint(); }
 
  private:
   Stream stream_;

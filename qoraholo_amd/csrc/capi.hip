@@ -161,7 +161,8 @@ BpmTables checked_bpm_tables(int nz, int n, int count, const qh_illumination* il
     require(illum[a].fy * illum[a].fy + illum[a].fx * illum[a].fx < fm * fm,
             w + ": illumination frequency beyond n_medium / wavelength");
   }
-  require(o->chunk >= 0 && o->checkpoint >= 0, w + ": chunk and checkpoint must not be negative");
+  require(o->chunk >= 0 && o->checkpoint >= -1,
+          w + ": chunk must not be negative, checkpoint at least -1");
   require_fft(n, who);
   return bpm_tables(n, nz, illum, count, *g, o->obliquity != 0);
 }
@@ -174,7 +175,8 @@ BpmTables tables_from_view(const qh_bpm_table_view* v, const qh_bpm_options* o, 
   require(v->n >= 4 && v->n % 2 == 0 && v->nz >= 1, w + ": volume must be (nz, n, n) with an even n");
   require(v->count >= 1, w + ": need one illumination per field");
   require(v->step != nullptr && v->last != nullptr && v->q != nullptr, w + ": missing table arrays");
-  require(o->chunk >= 0 && o->checkpoint >= 0, w + ": chunk and checkpoint must not be negative");
+  require(o->chunk >= 0 && o->checkpoint >= -1,
+          w + ": chunk must not be negative, checkpoint at least -1");
   require_fft(v->n, who);
   BpmTables t;
   t.n = v->n;
@@ -990,6 +992,13 @@ int qh_hip_bpm_plan_info(const qh_bpm_plan* plan, float* phase, float* lipschitz
     if (lipschitz) *lipschitz = plan->impl->lipschitz();
     if (chunk) *chunk = plan->impl->chunk();
     if (device_bytes) *device_bytes = plan->impl->bytes();
+  });
+}
+
+int qh_hip_bpm_plan_checkpoint(const qh_bpm_plan* plan, int* checkpoint) {
+  return guarded([&] {
+    require(plan != nullptr && checkpoint != nullptr, "bpm_plan: null plan or output");
+    *checkpoint = plan->impl->checkpoint();
   });
 }
 

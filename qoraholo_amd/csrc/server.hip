@@ -76,6 +76,7 @@ class BpmWorker final : public SlotWorker {
     inversion_.run(static_cast<const float2*>(input), out, extra);
   }
   int chunk() const { return inversion_.chunk(); }
+  int checkpoint() const { return inversion_.checkpoint(); }
 
  private:
   BpmInversion inversion_;
@@ -135,12 +136,14 @@ int Server::add_geometry(const qh_pipeline_options& o, const float* backgrounds,
 
 int Server::add_bpm(const BpmInversionSpec& spec) {
   auto shared = std::make_shared<BpmInversionSpec>(spec);
-  // An automatic illumination chunk is chosen from the free memory when the first slot is
-  // built (inside add_kind, before the geometry is visible to the dispatcher) and kept for every
-  // later slot, so that all slots have the footprint admission measured on the first.
+  // An automatic illumination chunk and checkpoint interval are chosen from the free memory
+  // when the first slot is built (inside add_kind, before the geometry is visible to the
+  // dispatcher) and kept for every later slot, so that all slots have the footprint admission
+  // measured on the first.
   return add_kind(kBpm, [shared] {
     auto w = std::make_unique<BpmWorker>(*shared);
     if (shared->chunk == 0) shared->chunk = w->chunk();
+    if (shared->checkpoint < 0) shared->checkpoint = w->checkpoint();
     return w;
   });
 }

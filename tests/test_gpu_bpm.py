@@ -77,9 +77,11 @@ def test_vjp_invariant_to_checkpoints_and_chunks(device, qh):
     g, il, _, x = case(qh, 32, 48)
     r = residual_pattern(il.shape[0], 48)
     ref = qh.bpm_vjp(x, r, il, g, checkpoint=0)
-    for checkpoint, chunk in ((16, 0), (5, 0), (1, 0), (0, 2), (7, 3)):
+    for checkpoint, chunk in ((16, 0), (5, 0), (1, 0), (0, 2), (7, 3), (-1, 0), (-1, 3)):
         assert np.array_equal(qh.bpm_vjp(x, r, il, g, checkpoint=checkpoint, chunk=chunk), ref), (
             checkpoint, chunk)
+    with pytest.raises(ValueError, match="checkpoint at least -1"):
+        qh.bpm_vjp(x, r, il, g, checkpoint=-2)
 
 
 @pytest.mark.parametrize("nz,n,count", [(16, 32, 5), (24, 48, 7), (12, 64, 4), (8, 96, 3),
@@ -260,14 +262,16 @@ def test_reconstruct_chunk_invariant(device, qh, oracle):
 
 def test_full_size_invariants(device, qh):
     """256^2 x 256 with 49 illuminations: the unit envelope passes through x = 0 exactly,
-    checkpointed and stored envelopes give the same adjoint bit for bit, and the adjoint is
-    linear in r."""
+    checkpointed and stored envelopes give the same adjoint bit for bit, the automatic
+    interval keeps every envelope (6.6 GB, far below half of the free memory), and the adjoint
+    is linear in r."""
     torch = pytest.importorskip("torch")
     g = qh.Geometry(*GEOM)
     il = qh.circular_scan(49, 0.9, g)
     n = nz = 256
     print("bpm full size 256^3 x 49", flush=True)
-    plan = qh.BpmPlan(nz, n, il, g)
+    plan = qh.BpmPlan(nz, n, il, g, checkpoint=16)
+    assert plan.checkpoint == 16
     zero = torch.zeros((nz, n, n), dtype=torch.float32, device="cuda")
     y0 = plan.forward(zero)
     assert torch.equal(y0, torch.ones_like(y0))
@@ -281,8 +285,13 @@ def test_full_size_invariants(device, qh):
     lin = float(torch.linalg.vector_norm(g12 - g1 - g2) / torch.linalg.vector_norm(g12))
     assert lin < 1e-5
     stored = qh.BpmPlan(nz, n, il, g, checkpoint=0)
+    assert stored.checkpoint == 0
     assert torch.equal(stored.vjp(x, r1), g1)
     del stored
+    auto = qh.BpmPlan(nz, n, il, g)
+    assert auto.checkpoint == 0 and auto.chunk == 49
+    assert torch.equal(auto.vjp(x, r1), g1)
+    del auto
     torch.cuda.synchronize()
 
 
@@ -310,7 +319,7 @@ def test_gradient_at_full_size(device, qh, oracle):
     x_true = (ph.render_phantom(overlapping_beads(ph), g, nz, n) - np.float32(GEOM[1]))
     x_true = x_true.astype(np.float32)
     print("bpm gradient at full size 256^3 x 49", flush=True)
-    plan = qh.BpmPlan(nz, n, il, g)
+    plan = qh.BpmPlan(nz, n, il, g, checkpoint=16)
     data = plan.forward(torch.from_numpy(x_true).cuda()).cpu().numpy()
     x = (0.7 * x_true).astype(np.float32)
     xt = torch.from_numpy(x).cuda()
