@@ -557,10 +557,12 @@ class Server:
     def add_bpm(self, n: int, illuminations, geometry: Geometry, nz: int = 0,
                 start_iterations: int = 20, iterations: int = 40, batch: int = 0,
                 step: float = 1.0, tv_weight: float = 0.0, tv_inner: int = 10,
-                obliquity: bool = True, chunk: int = 0, checkpoint: int = 16) -> int:
+                obliquity: bool = True, chunk: int = 0, checkpoint: int = -1) -> int:
         """Registers a BPM inversion geometry: field ratios (count, n, n) to a refractive index
         (nz, n, n) by ``bpm_reconstruct`` with these options, started from the
-        ``start_iterations`` Gerchberg-Papoulis reconstruction of their Rytov fields."""
+        ``start_iterations`` Gerchberg-Papoulis reconstruction of their Rytov fields. An
+        automatic ``chunk`` (0) and ``checkpoint`` (-1) are resolved from the free memory when
+        the geometry's first slot is built and kept for its later slots."""
         il = _illum(illuminations)
         count = il.shape[0]
         nzv = nz if nz > 0 else n
