@@ -101,6 +101,14 @@ struct BpmModel::Args {
   int s_zero;            // columns with s_in == null: the state in front of slice 0 is zero
                          // (the tangent), not the unit envelope
   const float* h;        // tangent direction of the slice, (n, n)
+  // Adjoint columns also sum the gradient terms of the slice processed before (red_p, left by
+  // the previous rows launch) into red_grad, in blocks after the line blocks; red_init: the
+  // first chunk, whose sum starts from zero rather than from red_grad.
+  const float* red_p;
+  float* red_grad;
+  int red_chunk;
+  int red_blocks;
+  int red_init;
 };
 
 namespace {
@@ -287,11 +295,37 @@ __global__ void __launch_bounds__(E::kBlock) k_bpm_rows_out(Args a) {
   e.store2([&](long l, int m, float2 v) { a.out[l * n + m] = v; });
 }
 
+// grad[i] += p[b][i] for b = 0 .. chunk-1 in order (from 0 instead of grad[i] with init, the
+// same bits as adding to a zeroed grad); the loads of eight illuminations are issued together
+// so that each thread keeps several in flight.
+__device__ __forceinline__ void reduce_terms(const float* p, int chunk, long plane, float* grad,
+                                             bool init, long i) {
+  float g = init ? 0.0f : grad[i];
+  int b = 0;
+  for (; b + 8 <= chunk; b += 8) {
+    float v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = p[(b + u) * plane + i];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) g += v[u];
+  }
+  for (; b < chunk; ++b) g += p[b * plane + i];
+  grad[i] = g;
+}
+
 // Adjoint columns: forward y transform completes back = FFT2(conj(t) b) of the slice above (or
 // FFT2(r) on the first slice processed); with the carried g, U = conj(c) (g + back), g <- (1-q) U,
 // b = q U (src/common.hpp:168-176); inverse y transform of b.
 template <class E>
 __global__ void __launch_bounds__(E::kBlock) k_bpm_cols_adj(Args a) {
+  if (a.red_blocks > 0 && static_cast<long>(blockIdx.x) >= a.g.blocks) {
+    // A reduction block (the last red_blocks of the grid, dealt to the XCDs in turn after the
+    // line blocks): the previous slice's gradient terms, one plane element per thread.
+    const long plane = static_cast<long>(a.g.n) * a.g.n;
+    const long i = (static_cast<long>(blockIdx.x) - a.g.blocks) * E::kBlock + threadIdx.x;
+    if (i < plane) reduce_terms(a.red_p, a.red_chunk, plane, a.red_grad, a.red_init != 0, i);
+    return;
+  }
   E e(a.g);
   const int n = a.g.n;
   const long plane = static_cast<long>(n) * n;
@@ -357,23 +391,11 @@ k_bpm_rows_adj(Args a) {
   e.store2([&](long l, int m, float2 v) { a.out[l * n + m] = v; });
 }
 
-// grad[i] += p[b][i] for b = 0 .. chunk-1 in order; the loads of eight illuminations are
-// issued together so that each thread keeps several in flight.
 __global__ void __launch_bounds__(256) k_bpm_reduce(const float* p, int chunk, long plane,
-                                                    float* grad) {
+                                                    float* grad, int init) {
   const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= plane) return;
-  float g = grad[i];
-  int b = 0;
-  for (; b + 8 <= chunk; b += 8) {
-    float v[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = p[(b + u) * plane + i];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) g += v[u];
-  }
-  for (; b < chunk; ++b) g += p[b * plane + i];
-  grad[i] = g;
+  reduce_terms(p, chunk, plane, grad, init != 0, i);
 }
 
 // exp(i phase x) per voxel (src/cpu/bpm.cpp:18-24).
@@ -439,6 +461,12 @@ void launch_with(Kind k, Args a, unsigned grid, std::size_t lds, hipStream_t s) 
       default: hipLaunchKernelGGL(k_bpm_rows_adj<E>, dim3(grid), dim3(E::kBlock), lds, s, a); break;
     }
   } else {
+    if (k == Kind::cols_adj && a.red_grad != nullptr) {
+      const long plane = static_cast<long>(a.g.n) * a.g.n;
+      a.g.blocks = static_cast<int>(grid);
+      a.red_blocks = static_cast<int>((plane + E::kBlock - 1) / E::kBlock);
+      grid += static_cast<unsigned>(a.red_blocks);
+    }
     switch (k) {
       case Kind::cols_fwd: hipLaunchKernelGGL(k_bpm_cols_fwd<E>, dim3(grid), dim3(E::kBlock), lds, s, a); break;
       default: hipLaunchKernelGGL(k_bpm_cols_adj<E>, dim3(grid), dim3(E::kBlock), lds, s, a); break;
@@ -621,9 +649,13 @@ void BpmModel::recompute(int seg, int a0, int chunk_count, hipStream_t s) {
   }
 }
 
-// Slices nz-1 .. 0 from the forward x transform of the residual in a_.
-void BpmModel::adjoint(int a0, int chunk_count, float* grad, hipStream_t s) {
+// Slices nz-1 .. 0 from the forward x transform of the residual in a_. The gradient terms a
+// rows launch leaves in p_ are summed into grad by the next slice's column launch (its
+// reduction blocks), those of slice 0 by a launch of their own; with init (the first chunk)
+// the sums start from zero, and grad needs no clearing.
+void BpmModel::adjoint(int a0, int chunk_count, float* grad, bool init, hipStream_t s) {
   const long cp = static_cast<long>(chunk_count) * plane_;
+  int pending = -1;  // slice whose terms are in p_, not yet summed
   for (int seg = nseg_ - 1; seg >= 0; --seg) {
     const int k0 = seg * seg_len_;
     const int k1 = std::min(k0 + seg_len_, nz_) - 1;
@@ -636,6 +668,12 @@ void BpmModel::adjoint(int a0, int chunk_count, float* grad, hipStream_t s) {
       c.first = k == nz_ - 1 ? 1 : 0;
       c.c = (k == nz_ - 1 ? last_.get() : step_.get()) + a0 * plane_;
       c.q = q_.get() + a0 * plane_;
+      if (pending >= 0) {
+        c.red_p = p_.get();
+        c.red_grad = grad + pending * plane_;
+        c.red_chunk = chunk_count;
+        c.red_init = init ? 1 : 0;
+      }
       launch<false>(Kind::cols_adj, c, s);
       Args r = base(chunk_count);
       r.in = a_.get();
@@ -644,11 +682,12 @@ void BpmModel::adjoint(int a0, int chunk_count, float* grad, hipStream_t s) {
       r.w = w_.get() + (k - k0) * cp;
       r.p = p_.get();
       launch<true>(Kind::rows_adj, r, s);
-      hipLaunchKernelGGL(k_bpm_reduce, dim3(blocks_for(plane_)), dim3(256), 0, s, p_.get(),
-                         chunk_count, plane_, grad + k * plane_);
-      check_launch("bpm_reduce");
+      pending = k;
     }
   }
+  hipLaunchKernelGGL(k_bpm_reduce, dim3(blocks_for(plane_)), dim3(256), 0, s, p_.get(),
+                     chunk_count, plane_, grad + pending * plane_, init ? 1 : 0);
+  check_launch("bpm_reduce");
 }
 
 void BpmModel::forward(const float* x, float2* y, hipStream_t s) {
@@ -666,12 +705,11 @@ void BpmModel::forward(const float* x, float2* y, hipStream_t s) {
 void BpmModel::vjp(const float* x, const float2* r, float* grad, hipStream_t s) {
   ensure_adjoint();
   set_screen(x, s);
-  fill(grad, static_cast<long>(nz_) * plane_, 0.0f, s);
   for (int a0 = 0; a0 < count_; a0 += chunk_) {
     const int b = std::min(chunk_, count_ - a0);
     forward_pass(a0, b, true, s);
     fft_lines(r + a0 * plane_, a_.get(), n_, lines_along(2, 1, b * n_, n_, n_), false, 1.0f, 0, s);
-    adjoint(a0, b, grad, s);
+    adjoint(a0, b, grad, a0 == 0, s);
   }
 }
 
@@ -724,7 +762,6 @@ void BpmModel::gradient(const float* x, const float2* fields, int first, int las
   require(first >= 0 && last <= count_ && first < last, "bpm: illumination range out of bounds");
   ensure_adjoint();
   set_screen(x, s);
-  fill(grad, static_cast<long>(nz_) * plane_, 0.0f, s);
   for (int a0 = first; a0 < last; a0 += chunk_) {
     const int b = std::min(chunk_, last - a0);
     forward_pass(a0, b, true, s);
@@ -734,7 +771,7 @@ void BpmModel::gradient(const float* x, const float2* fields, int first, int las
     r.data = fields + a0 * plane_;
     r.loss = loss_lines + static_cast<long>(a0) * n_;
     launch<true>(Kind::rows_out, r, s);
-    adjoint(a0, b, grad, s);
+    adjoint(a0, b, grad, a0 == first, s);
   }
 }
 

@@ -78,7 +78,7 @@ class BpmModel {
   void set_screen(const float* x, hipStream_t s);
   void forward_pass(int a0, int chunk_count, bool keep, hipStream_t s);
   void recompute(int seg, int a0, int chunk_count, hipStream_t s);
-  void adjoint(int a0, int chunk_count, float* grad, hipStream_t s);
+  void adjoint(int a0, int chunk_count, float* grad, bool init, hipStream_t s);
   float2* ck_s(int j) const;
   float2* ck_t(int j) const;
 

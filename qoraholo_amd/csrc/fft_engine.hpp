@@ -40,6 +40,7 @@ struct LineGeom {
   int n;
   long lines;
   int lines_per_block;  // LdsLines only
+  int blocks;           // the first `blocks` blocks of the grid run the lines (0: every block)
 };
 
 constexpr int ce_max(int a, int b) { return a > b ? a : b; }
@@ -96,7 +97,7 @@ struct RegLines {
     }
     line = LINE_FAST ? tid % LINES : tid / T;
     j = LINE_FAST ? tid / LINES : tid % T;
-    l = xcd_block() * LINES + line;
+    l = xcd_block(g.blocks > 0 ? g.blocks : static_cast<long>(gridDim.x)) * LINES + line;
     valid = l < g.lines;
   }
 
@@ -208,7 +209,7 @@ struct LdsLines {
     b1 = b0 + g.lines_per_block * n;
     cur = b0;
     for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = g.tw[i];
-    l0 = xcd_block() * g.lines_per_block;
+    l0 = xcd_block(g.blocks > 0 ? g.blocks : static_cast<long>(gridDim.x)) * g.lines_per_block;
     const long left = g.lines - l0;
     nl = static_cast<int>(left < g.lines_per_block ? left : g.lines_per_block);
     ls = CONTIG ? n : 1;

@@ -360,7 +360,9 @@ def test_served_request_at_384(device, qh, oracle):
     """The served BPM inversion at its larger size: 384^2 x 384 with 81 illuminations through
     the Server (Rytov, GP 20 start, 2 FISTA iterations with the TV prox of weight 1e-2). The
     served result equals the per-call chain bit for bit; the loss falls and n >= n_m; and the
-    device gradient at the start, on 2 of the 81 illuminations, matches the oracle's."""
+    device gradient at the start, on 2 of the 81 illuminations, matches the oracle's, with every
+    envelope stored (the automatic interval at this size) and, bit for bit, with checkpoints
+    every 16 slices."""
     torch = pytest.importorskip("torch")
     from qoraholo_amd import workloads
 
@@ -383,7 +385,12 @@ def test_served_request_at_384(device, qh, oracle):
 
     x = (start - nm).astype(np.float32)
     plan = qh.BpmPlan(384, 384, il, g)
-    grad, _ = plan.gradient(torch.from_numpy(x).cuda(), torch.from_numpy(f).cuda(), 40, 42)
+    assert plan.checkpoint == 0 and plan.chunk == 81  # 36.7 GB of envelopes fit
+    xt, ft = torch.from_numpy(x).cuda(), torch.from_numpy(f).cuda()
+    grad, _ = plan.gradient(xt, ft, 40, 42)
+    del plan
+    plan = qh.BpmPlan(384, 384, il, g, checkpoint=16)
+    assert torch.equal(plan.gradient(xt, ft, 40, 42)[0], grad)  # 24 recomputed segments
     del plan
     want, _ = oracle.bpm_gradient(x, f, il, GEOM, first=40, last=42)
     err = rel_l2(grad.cpu().numpy(), want)
