@@ -242,7 +242,13 @@ struct ZArgs {
   int lines_per_block;
   FftFactors f;
   const float2* tw;
+  long vol;  // elements between the volumes of a batch (0: one volume, lines == plane)
 };
+
+// First element of z line l: lines past a plane's belong to the next volumes of the batch.
+__device__ __forceinline__ long z_line_base(const ZArgs& a, long l) {
+  return a.vol == 0 ? l : (l / a.plane) * a.vol + l % a.plane;
+}
 
 __global__ void __launch_bounds__(kThreads) k_z_roundtrip_restore(ZArgs a) {
   extern __shared__ float2 smem[];
@@ -257,14 +263,14 @@ __global__ void __launch_bounds__(kThreads) k_z_roundtrip_restore(ZArgs a) {
   for (int e = threadIdx.x; e < total; e += blockDim.x) {
     const int i = e / nl;
     const int line = e - i * nl;
-    b0[e] = a.e[i * a.plane + l0 + line];
+    b0[e] = a.e[i * a.plane + z_line_base(a, l0 + line)];
   }
   __syncthreads();
   float2* r = lds_fft(b0, b1, 1, nl, nl, a.f, tw, false, true);
   for (int e = threadIdx.x; e < total; e += blockDim.x) {
     const int i = e / nl;
     const int line = e - i * nl;
-    const long g = i * a.plane + l0 + line;
+    const long g = i * a.plane + z_line_base(a, l0 + line);
     const float2 d = a.a[g];
     float2 v = r[e];
     hermitian_restore(d.x, d.y, a.hits[g], v.x, v.y);
@@ -275,7 +281,7 @@ __global__ void __launch_bounds__(kThreads) k_z_roundtrip_restore(ZArgs a) {
   for (int e = threadIdx.x; e < total; e += blockDim.x) {
     const int i = e / nl;
     const int line = e - i * nl;
-    a.e[i * a.plane + l0 + line] = r2[e];
+    a.e[i * a.plane + z_line_base(a, l0 + line)] = r2[e];
   }
 }
 
@@ -351,11 +357,12 @@ __global__ void __launch_bounds__((Shape4<N1, N2, true>::BLOCK)) k_fft4s_zrt(ZAr
   const int j = tid / S::LINES;
   const long l = xcd_block() * S::LINES + line;
   const bool valid = l < a.lines;
+  const long lb = z_line_base(a, l);
   float2 v[N2];
   float2 w[N1];
 #pragma unroll
   for (int n2 = 0; n2 < N2; ++n2) {
-    v[n2] = (valid && j < N1) ? a.e[static_cast<long>(j + N1 * n2) * a.plane + l] : float2{0.0f, 0.0f};
+    v[n2] = (valid && j < N1) ? a.e[static_cast<long>(j + N1 * n2) * a.plane + lb] : float2{0.0f, 0.0f};
   }
   __syncthreads();
   fft_4step<N1, N2, S::LINES, true>(v, w, j, line, t, tw, false);
@@ -367,17 +374,17 @@ __global__ void __launch_bounds__((Shape4<N1, N2, true>::BLOCK)) k_fft4s_zrt(ZAr
     if constexpr (N1 == 16 && (N2 == 16 || N2 == 24)) {
       std::uint8_t h[N1];
 #pragma unroll
-      for (int k1 = 0; k1 < N1; ++k1) h[k1] = a.hits[static_cast<long>(N2 * k1 + j) * a.plane + l];
+      for (int k1 = 0; k1 < N1; ++k1) h[k1] = a.hits[static_cast<long>(N2 * k1 + j) * a.plane + lb];
 #pragma unroll
       for (int k1 = 0; k1 < N1; ++k1) {
-        const long g = static_cast<long>(N2 * k1 + j) * a.plane + l;
+        const long g = static_cast<long>(N2 * k1 + j) * a.plane + lb;
         const float2 d = h[k1] != 0 ? a.a[g] : float2{0.0f, 0.0f};
         hermitian_restore(d.x, d.y, h[k1], w[k1].x, w[k1].y);
       }
     } else {
 #pragma unroll
       for (int k1 = 0; k1 < N1; ++k1) {
-        const long g = static_cast<long>(N2 * k1 + j) * a.plane + l;
+        const long g = static_cast<long>(N2 * k1 + j) * a.plane + lb;
         const float2 d = a.a[g];
         hermitian_restore(d.x, d.y, a.hits[g], w[k1].x, w[k1].y);
       }
@@ -388,7 +395,7 @@ __global__ void __launch_bounds__((Shape4<N1, N2, true>::BLOCK)) k_fft4s_zrt(ZAr
   fft_4step<N2, N1, S::LINES, true>(w, u, j, line, t, tw, true);
   if (valid && j < N1) {
 #pragma unroll
-    for (int k1 = 0; k1 < N2; ++k1) a.e[static_cast<long>(N1 * k1 + j) * a.plane + l] = u[k1];
+    for (int k1 = 0; k1 < N2; ++k1) a.e[static_cast<long>(N1 * k1 + j) * a.plane + lb] = u[k1];
   }
 }
 
@@ -687,16 +694,17 @@ void irfft_rows(const float2* in, long in_pitch, float* x, long x_pitch, long ro
 }
 
 void z_roundtrip_restore(float2* e, const float2* a, const std::uint8_t* hits, int nz, int ny,
-                         int h, hipStream_t s) {
+                         int h, hipStream_t s, int batch) {
+  require(batch >= 1, "z_roundtrip_restore: batch must be positive");
   const long plane = static_cast<long>(ny) * h;
-  ZArgs z{e, a, hits, plane, plane, nz, strided_lines_per_block(nz), fft_factors(nz),
-          fft_twiddles(nz)};
+  ZArgs z{e, a, hits, plane * batch, plane, nz, strided_lines_per_block(nz), fft_factors(nz),
+          fft_twiddles(nz), batch > 1 ? plane * nz : 0};
   if (with_four_step<LaunchZrt>(nz, z, s)) {
     check_launch("z_roundtrip_restore");
     return;
   }
   const std::size_t lds = (nz + 2 * static_cast<std::size_t>(z.lines_per_block) * nz) * sizeof(float2);
-  const unsigned grid = static_cast<unsigned>((plane + z.lines_per_block - 1) / z.lines_per_block);
+  const unsigned grid = static_cast<unsigned>((z.lines + z.lines_per_block - 1) / z.lines_per_block);
   hipLaunchKernelGGL(k_z_roundtrip_restore, dim3(grid), dim3(kThreads), lds, s, z);
   check_launch("z_roundtrip_restore");
 }

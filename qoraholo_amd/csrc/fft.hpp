@@ -68,11 +68,11 @@ void irfft_rows(const float2* in, long in_pitch, float* x, long x_pitch, long ro
                 const RealOp& op, hipStream_t s);
 
 // Half-spectrum solver step along z: forward transform of each z line of e, restore
-// e <- a + (1 - hits / 2) e, inverse transform. e, a, hits are (nz, ny, h) arrays. a must be
-// +0 (both parts) wherever hits is 0, as the folded Hermitian data is: lengths 256 and 384 do
-// not read a there.
+// e <- a + (1 - hits / 2) e, inverse transform. e, a, hits are (nz, ny, h) arrays, or `batch`
+// of them one after another. a must be +0 (both parts) wherever hits is 0, as the folded
+// Hermitian data is: lengths 256 and 384 do not read a there.
 void z_roundtrip_restore(float2* e, const float2* a, const std::uint8_t* hits, int nz, int ny,
-                         int h, hipStream_t s);
+                         int h, hipStream_t s, int batch = 1);
 
 // 3D real <-> half-complex transforms over dense (nz, ny, nx) real arrays and (nz, ny, nx/2+1)
 // complex arrays. The inverse scales by `scale` and may overwrite its input.

@@ -176,8 +176,8 @@ __global__ void k_index_complex(const float2* p, float* ri, long total, float nm
 
 }  // namespace
 
-OdtSolver::OdtSolver(const OdtGrid& g, int count, const OdtConfig& c)
-    : g_(g), count_(count), c_(c),
+OdtSolver::OdtSolver(const OdtGrid& g, int count, const OdtConfig& c, int batch)
+    : g_(g), count_(count), batch_(batch), c_(c),
       real_(c.iterations > 0 && (c.solver == 1 || c.gp_transform == 0)),
       tv_(c.iterations > 0 && c.solver == 1),
       vol_(static_cast<long>(g.nz) * g.n * g.n),
@@ -185,6 +185,7 @@ OdtSolver::OdtSolver(const OdtGrid& g, int count, const OdtConfig& c)
   require(fft_supported(g.n) && fft_supported(g.nz), "reconstruct: sizes must factor into 2, 3, 5");
   require(count <= 65535, "reconstruct: at most 65535 illuminations");
   require(half_ < (1L << 31), "reconstruct: at most 2^31 half-spectrum bins");
+  require(batch >= 1, "reconstruct: the batch must be positive");
   const float k0 = 2.0f * kPi / c.wavelength;
   nm2_ = c.n_medium * c.n_medium;
   inv_k02_ = 1.0f / (k0 * k0);
@@ -195,10 +196,13 @@ OdtSolver::OdtSolver(const OdtGrid& g, int count, const OdtConfig& c)
   ri_.resize(vol_);
   filled_.resize(1);
   if (real_) {
-    a_.resize(half_);
-    hits_.resize(half_);
-    e_.resize(half_);
-    x_[0].resize(vol_);
+    // A batched Gerchberg-Papoulis solve keeps the data, hits, spectrum and iterate of every
+    // request of the batch.
+    const long nb = batchable() ? batch : 1;
+    a_.resize(nb * half_);
+    hits_.resize(nb * half_);
+    e_.resize(nb * half_);
+    x_[0].resize(nb * vol_);
     x_[1].resize(vol_);
     if (tv_) {
       x_[2].resize(vol_);
@@ -230,7 +234,8 @@ std::size_t OdtSolver::bytes() const {
   return b;
 }
 
-void OdtSolver::map(float2* fields, hipStream_t s, StageClock* clock, int first, bool fold) {
+void OdtSolver::map(float2* fields, hipStream_t s, StageClock* clock, int first, bool fold,
+                    int slot) {
   const auto mark = [&](int m) {
     if (clock) clock->mark(first + m, s);
   };
@@ -250,7 +255,8 @@ void OdtSolver::map(float2* fields, hipStream_t s, StageClock* clock, int first,
   if (fold) {
     hipLaunchKernelGGL(k_fold_half, dim3(std::min(blocks_for(half_), kCountBlocks)), dim3(256), 0, s,
                        potential_.get(),
-                       counts_.get(), g_.nz, g_.n, a_.get(), hits_.get(), filled_.get());
+                       counts_.get(), g_.nz, g_.n, a_.get() + slot * half_,
+                       hits_.get() + slot * half_, filled_.get());
     check_launch("fold_half");
   } else {
     hipLaunchKernelGGL(k_normalize, dim3(std::min(blocks_for(vol_), kCountBlocks)), dim3(256), 0, s,
@@ -259,6 +265,43 @@ void OdtSolver::map(float2* fields, hipStream_t s, StageClock* clock, int first,
     check_launch("normalize");
   }
   mark(kNormalize);
+}
+
+void OdtSolver::run_batch(float2* fields, long field_stride, int nb, float* out, hipStream_t s) {
+  require(batchable(), "reconstruct: only the Gerchberg-Papoulis solve on half spectra batches");
+  require(nb >= 1 && nb <= batch_, "reconstruct: batch beyond the solver's capacity");
+  for (int b = 0; b < nb; ++b) map(fields + b * field_stride, s, nullptr, 0, true, b);
+  const int n = g_.n;
+  const int nz = g_.nz;
+  const int h = n / 2 + 1;
+  const float inv = 1.0f / static_cast<float>(vol_);
+  // Volumes lie one after another, so the row and y passes see one array of nb * nz planes;
+  // the z passes take the batch.
+  const long rows = static_cast<long>(nb) * nz * n;
+  for (int b = 0; b < nb; ++b) {
+    fft_lines(a_.get() + b * half_, e_.get() + b * half_, nz, lines_along(0, nz, n, h, h), true,
+              1.0f, 0, s);
+  }
+  fft_lines(e_.get(), e_.get(), n, lines_along(1, nb * nz, n, h, h), true, 1.0f, 0, s);
+  RealOp first_op;
+  first_op.a = inv;
+  irfft_rows(e_.get(), h, x_[0].get(), n, rows, n, first_op, s);
+  for (int it = 0; it < c_.iterations; ++it) {
+    const bool last = it == c_.iterations - 1;
+    RealOp op;
+    op.in = RealIn::nonneg;
+    op.a = 0.0f;
+    rfft_rows(x_[0].get(), nullptr, n, e_.get(), h, rows, n, op, s);
+    fft_lines(e_.get(), e_.get(), n, lines_along(1, nb * nz, n, h, h), false, 1.0f, 0, s);
+    z_roundtrip_restore(e_.get(), a_.get(), hits_.get(), nz, n, h, s, nb);
+    fft_lines(e_.get(), e_.get(), n, lines_along(1, nb * nz, n, h, h), true, 1.0f, 0, s);
+    RealOp oo;
+    oo.out = last ? RealOut::index : RealOut::scale;
+    oo.a = inv;
+    oo.nm2 = nm2_;
+    oo.inv_k02 = inv_k02_;
+    irfft_rows(e_.get(), h, last ? out : x_[0].get(), n, rows, n, oo, s);
+  }
 }
 
 double OdtSolver::collect_fgp_ms() {

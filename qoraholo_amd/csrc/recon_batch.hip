@@ -30,7 +30,7 @@ ReconBatch::ReconBatch(const qh_pipeline_options& o, const float* host_backgroun
       cap_(capacity),
       crop_{o.crop.cy, o.crop.cx, o.crop.radius, o.crop.out},
       unwrapper_(batch_images(o, capacity), o.crop.out, o.crop.out),
-      solver_(grid_from(o), o.count, config_from(o.odt, o.geometry)),
+      solver_(grid_from(o), o.count, config_from(o.odt, o.geometry), capacity),
       holo_(static_cast<std::size_t>(capacity) * o.count * o.hologram_size * o.hologram_size),
       half_(static_cast<std::size_t>(capacity) * o.count * o.hologram_size *
             (o.hologram_size / 2 + 1)),
@@ -86,7 +86,8 @@ ReconBatch::ReconBatch(const qh_pipeline_options& o, const float* host_backgroun
 
 // The device work of nb requests whose holograms are in holo_: the front end of every stack in
 // one set of launches (the background fields repeat over the stacks), then each request's
-// mapping and solve, its index volume into its own part of ri_.
+// mapping and, for Gerchberg-Papoulis, one solve over all nb volumes (FISTA-TV: each request's
+// solve in turn), each index volume into its own part of ri_.
 void ReconBatch::device_work(int nb, hipStream_t s) {
   const int images = nb * count_;
   const std::size_t field = static_cast<std::size_t>(m_) * m_;
@@ -97,6 +98,10 @@ void ReconBatch::device_work(int nb, hipStream_t s) {
   ratio_phase(fields_.get(), background_.get(), static_cast<long>(images) * field, psi_.get(),
               logamp_.get(), s, static_cast<long>(count_) * field);
   unwrapper_.rytov(psi_.get(), logamp_.get(), fields_.get(), s, images);
+  if (solver_.batchable()) {
+    solver_.run_batch(fields_.get(), static_cast<long>(count_) * field, nb, ri_.get(), s);
+    return;
+  }
   for (int b = 0; b < nb; ++b) {
     solver_.set_output(ri_.get() + b * vol);
     solver_.run(fields_.get() + static_cast<std::size_t>(b) * count_ * field, s);

@@ -106,11 +106,19 @@ class StageClock {
 
 class OdtSolver {
  public:
-  OdtSolver(const OdtGrid& g, int count, const OdtConfig& c);
+  // `batch`: the requests run_batch takes at once (the real Gerchberg-Papoulis solve only).
+  OdtSolver(const OdtGrid& g, int count, const OdtConfig& c, int batch = 1);
   void set_illuminations(const float* illum_host);  // (count, 2) fy, fx
   // fields: (count, n, n) complex on the device, overwritten by their spectra. The index volume
   // (nz, n, n) is left in index(). Marks go to clock[first + OdtMark] when given.
   void run(float2* fields, hipStream_t s, StageClock* clock = nullptr, int first = 0);
+  // Whether run_batch applies: the Gerchberg-Papoulis solve on half spectra.
+  bool batchable() const { return real_ && !tv_; }
+  // nb <= batch requests whose fields are `field_stride` elements apart: each is mapped on its
+  // own, then their solves run together, every launch over all nb volumes (the same operations
+  // per line and per row as run(), so each volume has the bits run() gives it). The index
+  // volumes go to out, (nb, nz, n, n).
+  void run_batch(float2* fields, long field_stride, int nb, float* out, hipStream_t s);
   // Normalised k-space potential and hit counts after the mapping (for intermediate checks).
   void map_only(float2* fields, hipStream_t s);
   const float* index() const { return ri_.get(); }
@@ -133,9 +141,10 @@ class OdtSolver {
   bool timing_fgp() const { return time_fgp_; }
 
  private:
-  void map(float2* fields, hipStream_t s, StageClock* clock, int first, bool fold);
+  void map(float2* fields, hipStream_t s, StageClock* clock, int first, bool fold, int slot = 0);
   OdtGrid g_;
   int count_;
+  int batch_;
   OdtConfig c_;
   bool real_;
   bool tv_;
