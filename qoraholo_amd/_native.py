@@ -176,6 +176,8 @@ _SIGNATURES = {
     "qh_hip_rfft3": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p]),
     "qh_hip_irfft3": (c_int, [c_void_p, c_int, c_int, c_int, c_float, c_void_p]),
     "qh_hip_tv_prox": (c_int, [c_void_p, c_int, c_int, c_int, c_float, c_int, c_int, c_void_p]),
+    "qh_hip_tv_prox_batch": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_float, c_int, c_int,
+                                     c_void_p]),
     "qh_hip_tv_prox_dev_noalloc": (c_int, [c_void_p, c_int, c_int, c_int, c_float, c_int, c_int,
                                            c_void_p, c_void_p]),
 }

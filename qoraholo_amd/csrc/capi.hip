@@ -1074,4 +1074,24 @@ int qh_hip_tv_prox(const float* b, int nz, int ny, int nx, float lambda, int inn
   });
 }
 
+int qh_hip_tv_prox_batch(const float* b, int batch, int nz, int ny, int nx, float lambda,
+                         int inner, int fused, float* x) {
+  return guarded([&] {
+    require(b && x && batch >= 1 && nz >= 1 && ny >= 1 && nx >= 1 && lambda > 0 && inner >= 0,
+            "tv_prox_batch: bad arguments");
+    require_device();
+    const hipStream_t s = call_stream();
+    const std::size_t vol = static_cast<std::size_t>(nz) * ny * nx;
+    DeviceBuffer<float> db(vol * batch);
+    DeviceBuffer<float> dx(vol * batch);
+    TvState state(nz, ny, nx, batch);
+    state.set_fused(fused != 0);
+    state.reset(s, batch);
+    upload(db.get(), b, db.bytes(), s);
+    state.prox(db.get(), dx.get(), lambda, inner, s);
+    download(x, dx.get(), dx.bytes(), s);
+    check(hipStreamSynchronize(s), "tv_prox_batch");
+  });
+}
+
 }  // extern "C"

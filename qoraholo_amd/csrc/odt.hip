@@ -196,17 +196,17 @@ OdtSolver::OdtSolver(const OdtGrid& g, int count, const OdtConfig& c, int batch)
   ri_.resize(vol_);
   filled_.resize(1);
   if (real_) {
-    // A batched Gerchberg-Papoulis solve keeps the data, hits, spectrum and iterate of every
-    // request of the batch.
-    const long nb = batchable() ? batch : 1;
+    // A batched solve keeps the data, hits, spectrum, iterates and TV duals of every request of
+    // the batch.
+    const long nb = batch;
     a_.resize(nb * half_);
     hits_.resize(nb * half_);
     e_.resize(nb * half_);
     x_[0].resize(nb * vol_);
-    x_[1].resize(vol_);
+    x_[1].resize(nb * vol_);
     if (tv_) {
-      x_[2].resize(vol_);
-      tv_state_ = std::make_unique<TvState>(g.nz, g.n, g.n);
+      x_[2].resize(nb * vol_);
+      tv_state_ = std::make_unique<TvState>(g.nz, g.n, g.n, batch);
       fgp_events_.resize(2 * c.iterations);
     }
   } else if (c.iterations > 0) {
@@ -268,7 +268,7 @@ void OdtSolver::map(float2* fields, hipStream_t s, StageClock* clock, int first,
 }
 
 void OdtSolver::run_batch(float2* fields, long field_stride, int nb, float* out, hipStream_t s) {
-  require(batchable(), "reconstruct: only the Gerchberg-Papoulis solve on half spectra batches");
+  require(batchable(), "reconstruct: only the solves on half spectra batch");
   require(nb >= 1 && nb <= batch_, "reconstruct: batch beyond the solver's capacity");
   for (int b = 0; b < nb; ++b) map(fields + b * field_stride, s, nullptr, 0, true, b);
   const int n = g_.n;
@@ -286,21 +286,49 @@ void OdtSolver::run_batch(float2* fields, long field_stride, int nb, float* out,
   RealOp first_op;
   first_op.a = inv;
   irfft_rows(e_.get(), h, x_[0].get(), n, rows, n, first_op, s);
-  for (int it = 0; it < c_.iterations; ++it) {
-    const bool last = it == c_.iterations - 1;
+  // One gradient step of every volume, as run()'s gradient_step.
+  const auto gradient_step = [&](const float* x, const float* x2, RealIn in, float m, float* v,
+                                 RealOut kind) {
     RealOp op;
-    op.in = RealIn::nonneg;
-    op.a = 0.0f;
-    rfft_rows(x_[0].get(), nullptr, n, e_.get(), h, rows, n, op, s);
+    op.in = in;
+    op.a = m;
+    rfft_rows(x, x2, n, e_.get(), h, rows, n, op, s);
     fft_lines(e_.get(), e_.get(), n, lines_along(1, nb * nz, n, h, h), false, 1.0f, 0, s);
     z_roundtrip_restore(e_.get(), a_.get(), hits_.get(), nz, n, h, s, nb);
     fft_lines(e_.get(), e_.get(), n, lines_along(1, nb * nz, n, h, h), true, 1.0f, 0, s);
     RealOp oo;
-    oo.out = last ? RealOut::index : RealOut::scale;
+    oo.out = kind;
     oo.a = inv;
     oo.nm2 = nm2_;
     oo.inv_k02 = inv_k02_;
-    irfft_rows(e_.get(), h, last ? out : x_[0].get(), n, rows, n, oo, s);
+    irfft_rows(e_.get(), h, v, n, rows, n, oo, s);
+  };
+  if (!tv_) {
+    for (int it = 0; it < c_.iterations; ++it) {
+      const bool last = it == c_.iterations - 1;
+      gradient_step(x_[0].get(), nullptr, RealIn::nonneg, 0.0f, last ? out : x_[0].get(),
+                    last ? RealOut::index : RealOut::scale);
+    }
+    return;
+  }
+  // FISTA-TV as run() does it, every step over the batch; the TV prox takes the batch too.
+  tv_state_->reset(s, nb);
+  int cur = 0;
+  int prev = 0;
+  float t = 1.0f;
+  float m = 0.0f;
+  float* v = x_[2].get();
+  for (int it = 0; it < c_.iterations; ++it) {
+    const bool last = it == c_.iterations - 1;
+    gradient_step(x_[cur].get(), x_[prev].get(), RealIn::momentum, m, v, RealOut::scale);
+    const int next = prev != cur ? prev : 1 - cur;
+    tv_state_->prox(v, last ? out : x_[next].get(), lambda_, c_.tv_inner, s, last, nm2_,
+                    inv_k02_);
+    prev = cur;
+    cur = next;
+    const float tn = fista_t(t);
+    m = (t - 1.0f) / tn;
+    t = tn;
   }
 }
 

@@ -86,8 +86,8 @@ ReconBatch::ReconBatch(const qh_pipeline_options& o, const float* host_backgroun
 
 // The device work of nb requests whose holograms are in holo_: the front end of every stack in
 // one set of launches (the background fields repeat over the stacks), then each request's
-// mapping and, for Gerchberg-Papoulis, one solve over all nb volumes (FISTA-TV: each request's
-// solve in turn), each index volume into its own part of ri_.
+// mapping and, on half spectra, one solve over all nb volumes (complex-spectrum solves: each
+// request's in turn), each index volume into its own part of ri_.
 void ReconBatch::device_work(int nb, hipStream_t s) {
   const int images = nb * count_;
   const std::size_t field = static_cast<std::size_t>(m_) * m_;

@@ -1,8 +1,8 @@
 // A reconstruction slot of the served loop that runs several requests of one geometry together:
 // the front end (hologram transforms, crop, field inverse, ratio, unwrap, Rytov) takes every
 // hologram stack of the batch in one set of launches, then each request's Ewald mapping runs in
-// turn, and the Gerchberg-Papoulis solves of all of them run as one (every transform and
-// elementwise launch over all the batch's volumes; FISTA-TV solves run in turn). The device work of a batch of b requests is one
+// turn, and the Gerchberg-Papoulis or FISTA-TV solves of all of them run as one (every
+// transform, elementwise and TV launch over all the batch's volumes). The device work of a batch of b requests is one
 // hipGraph, captured for every b up to the capacity when the slot is built. Every result equals
 // what the request gets alone on a Pipeline, bit for bit: each image and each volume goes
 // through the same kernels with the same operations, only more of them per launch.

@@ -106,14 +106,14 @@ class StageClock {
 
 class OdtSolver {
  public:
-  // `batch`: the requests run_batch takes at once (the real Gerchberg-Papoulis solve only).
+  // `batch`: the requests run_batch takes at once (the solves on half spectra).
   OdtSolver(const OdtGrid& g, int count, const OdtConfig& c, int batch = 1);
   void set_illuminations(const float* illum_host);  // (count, 2) fy, fx
   // fields: (count, n, n) complex on the device, overwritten by their spectra. The index volume
   // (nz, n, n) is left in index(). Marks go to clock[first + OdtMark] when given.
   void run(float2* fields, hipStream_t s, StageClock* clock = nullptr, int first = 0);
-  // Whether run_batch applies: the Gerchberg-Papoulis solve on half spectra.
-  bool batchable() const { return real_ && !tv_; }
+  // Whether run_batch applies: the Gerchberg-Papoulis and FISTA-TV solves on half spectra.
+  bool batchable() const { return real_; }
   // nb <= batch requests whose fields are `field_stride` elements apart: each is mapped on its
   // own, then their solves run together, every launch over all nb volumes (the same operations
   // per line and per row as run(), so each volume has the bits run() gives it). The index

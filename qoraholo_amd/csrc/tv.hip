@@ -39,6 +39,14 @@ struct TvArgs {
   int zlen = kZChunk;  // planes per chunk of the strip kernels (strip_grid)
 };
 
+// Volume blockIdx.y of a batch of volumes that lie one after another: its primal data and
+// output start blockIdx.y vol floats in, its duals (three components each) 3 blockIdx.y vol in.
+__device__ __forceinline__ long batch_volume() { return static_cast<long>(blockIdx.y); }
+template <class T>
+__device__ __forceinline__ T* shifted(T* p, long by) {
+  return p == nullptr ? p : p + by;
+}
+
 // Block b runs tile t: with a grid that is a multiple of 8, blocks b, b+8, b+16, ... (which the
 // dispatcher places on the same XCD) get consecutive tiles, so the halo rows and the planes
 // shared by neighbouring tiles are read through the same L2.
@@ -219,6 +227,10 @@ __device__ __forceinline__ void load_plane(const TvArgs& a, int z, int y0, int x
 
 template <int VX, int TY, bool VEC>
 __global__ void __launch_bounds__(64) k_fgp_step(TvArgs a, int chunks) {
+  a.b = shifted(a.b, batch_volume() * a.vol);
+  a.p_in = shifted(a.p_in, 3 * batch_volume() * a.vol);
+  a.p_prev = shifted(a.p_prev, 3 * batch_volume() * a.vol);
+  a.p_out = shifted(a.p_out, 3 * batch_volume() * a.vol);
   const int t = xcd_tile();
   const int y0 = (t / chunks) * TY;
   const int z0 = (t % chunks) * a.zlen;
@@ -480,6 +492,11 @@ __device__ __forceinline__ void lds_get(const float* buf, int row, int nx, int x
 // VEC: see load_row. XT: rows are cut into x tiles (more than one per row).
 template <int VX, int NW, bool VEC, bool XT>
 __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
+  a.b = shifted(a.b, batch_volume() * a.vol);
+  a.p_k = shifted(a.p_k, 3 * batch_volume() * a.vol);
+  a.p_km1 = shifted(a.p_km1, 3 * batch_volume() * a.vol);
+  a.p_k1 = shifted(a.p_k1, 3 * batch_volume() * a.vol);
+  a.p_k2 = shifted(a.p_k2, 3 * batch_volume() * a.vol);
   constexpr int TOP = 2;  // halo rows above the output rows (and two below)
   constexpr int T = NW - 4;
   // LDS row width: the row when a wave holds it whole, else the columns one wave holds.
@@ -698,6 +715,9 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
 template <int VX, int TY, bool VEC>
 __global__ void __launch_bounds__(64) k_tv_primal(TvArgs a, int chunks, float* out, int to_index,
                                                   float nm2, float inv_k02) {
+  a.b = shifted(a.b, batch_volume() * a.vol);
+  a.p_in = shifted(a.p_in, 3 * batch_volume() * a.vol);
+  out += batch_volume() * a.vol;
   const int t = xcd_tile();
   const int y0 = (t / chunks) * TY;
   const int z0 = (t % chunks) * a.zlen;
@@ -792,14 +812,14 @@ int device_cus() {
 // One wave per strip of kRows rows and z chunk: chunks of kZChunk planes, halved (down to 2)
 // while the grid holds fewer than two waves per SIMD (128^3: 4 planes and 2048 waves, where
 // 16-plane chunks gave 512 waves for 1024 SIMDs).
-dim3 strip_grid(int ny, int nz, TvArgs& a, int* chunks) {
+dim3 strip_grid(int ny, int nz, TvArgs& a, int* chunks, int batch) {
   const long strips = (ny + kRows - 1) / kRows;
   const long want = 8L * device_cus();
   int zlen = kZChunk;
-  while (zlen > 2 && strips * ((nz + zlen - 1) / zlen) < want) zlen /= 2;
+  while (zlen > 2 && strips * ((nz + zlen - 1) / zlen) * batch < want) zlen /= 2;
   a.zlen = zlen;
   *chunks = (nz + zlen - 1) / zlen;
-  return dim3(static_cast<unsigned>(strips * *chunks));
+  return dim3(static_cast<unsigned>(strips * *chunks), static_cast<unsigned>(batch));
 }
 
 // Whole-vector row access (see load_row) when the row splits into whole lane chunks.
@@ -837,11 +857,12 @@ void launch_primal(const TvArgs& a, int chunks, dim3 grid, float* out, int ti, f
 
 void fgp_step(const float* b, const float* p_in, const float* p_prev, float* p_out, int nz,
               int ny, int nx, float lambda, float step, float momentum, bool first,
-              hipStream_t s) {
+              hipStream_t s, int batch) {
+  require(batch >= 1, "fgp_step: batch must be positive");
   TvArgs a{b, p_in, p_prev, p_out, static_cast<long>(nz) * ny * nx, nz, ny, nx,
            lambda, step, momentum, first ? 1 : 0};
   int chunks = 0;
-  const dim3 grid = strip_grid(ny, nz, a, &chunks);
+  const dim3 grid = strip_grid(ny, nz, a, &chunks, batch);
   switch (lanes_x(nx)) {
     case 1: launch_step<1>(a, chunks, grid, s); break;
     case 2: launch_step<2>(a, chunks, grid, s); break;
@@ -853,11 +874,12 @@ void fgp_step(const float* b, const float* p_in, const float* p_prev, float* p_o
 }
 
 void tv_primal(const float* b, const float* p, int nz, int ny, int nx, float lambda, float* out,
-               bool to_index, float nm2, float inv_k02, hipStream_t s) {
+               bool to_index, float nm2, float inv_k02, hipStream_t s, int batch) {
+  require(batch >= 1, "tv_primal: batch must be positive");
   TvArgs a{b, p, nullptr, nullptr, static_cast<long>(nz) * ny * nx, nz, ny, nx,
            lambda, 0.0f, 0.0f, 1};
   int chunks = 0;
-  const dim3 grid = strip_grid(ny, nz, a, &chunks);
+  const dim3 grid = strip_grid(ny, nz, a, &chunks, batch);
   const int ti = to_index ? 1 : 0;
   switch (lanes_x(nx)) {
     case 1: launch_primal<1>(a, chunks, grid, out, ti, nm2, inv_k02, s); break;
@@ -911,28 +933,30 @@ int pair_chunk(const void* kernel, std::atomic<int>& per_cu_cache, int threads, 
   return best;
 }
 
+// Volumes of a batch are the grid's second dimension; their blocks count in the chunk choice.
 template <int VX, int NW, bool VEC, bool XT>
-void launch_pair_kernel(PairArgs a, int tiles, std::size_t lds, hipStream_t s) {
+void launch_pair_kernel(PairArgs a, int tiles, std::size_t lds, hipStream_t s, int batch) {
   // Blocks per CU of this kernel (its registers and LDS; the same on every gfx950 device).
   static std::atomic<int> per_cu{0};
   a.zlen = pair_chunk(reinterpret_cast<const void*>(&k_fgp_pair<VX, NW, VEC, XT>), per_cu,
-                      NW * 64, lds, tiles, a.nz);
+                      NW * 64, lds, tiles * batch, a.nz);
   const int chunks = (a.nz + a.zlen - 1) / a.zlen;
-  hipLaunchKernelGGL((k_fgp_pair<VX, NW, VEC, XT>), dim3(static_cast<unsigned>(tiles * chunks)),
+  hipLaunchKernelGGL((k_fgp_pair<VX, NW, VEC, XT>),
+                     dim3(static_cast<unsigned>(tiles * chunks), static_cast<unsigned>(batch)),
                      dim3(NW * 64), lds, s, a, chunks);
 }
 
 template <int VX, int NW, bool XT>
-void launch_pair(const PairArgs& a, int tiles, hipStream_t s) {
+void launch_pair(const PairArgs& a, int tiles, hipStream_t s, int batch) {
   // Four row buffers of NW rows of nx floats (64 VX per x tile): at most 48 KiB.
   const std::size_t lds = 4 * static_cast<std::size_t>(NW) * (XT ? 64 * VX : a.nx) * sizeof(float);
   if constexpr (VX % 2 == 0) {
     if (rows_vec<VX>(a.nx)) {
-      launch_pair_kernel<VX, NW, true, XT>(a, tiles, lds, s);
+      launch_pair_kernel<VX, NW, true, XT>(a, tiles, lds, s, batch);
       return;
     }
   }
-  launch_pair_kernel<VX, NW, false, XT>(a, tiles, lds, s);
+  launch_pair_kernel<VX, NW, false, XT>(a, tiles, lds, s, batch);
 }
 
 }  // namespace
@@ -941,8 +965,9 @@ bool fgp_pair_supported(int nx) { return nx >= 1 && nx <= 512; }
 
 void fgp_pair(const float* b, const float* p_k, const float* p_km1, float* p_k1, float* p_k2,
               int nz, int ny, int nx, float lambda, float step, float m1, bool first, float m2,
-              hipStream_t s) {
+              hipStream_t s, int batch) {
   require(fgp_pair_supported(nx), "fgp_pair: rows up to 512 wide");
+  require(batch >= 1, "fgp_pair: batch must be positive");
   int vx = std::min(lanes_x(nx), 4);
   int tiles_x = 1;
   int xt = nx;
@@ -960,15 +985,15 @@ void fgp_pair(const float* b, const float* p_k, const float* p_km1, float* p_k1,
              nz, ny,     nx,    lambda, step,        m1, first ? 1 : 0, m2, 0, xt, tiles_x};
   if (tiles_x > 1) {
     if (vx == 4) {
-      launch_pair<4, 12, true>(a, tiles, s);
+      launch_pair<4, 12, true>(a, tiles, s, batch);
     } else {
-      launch_pair<2, 16, true>(a, tiles, s);
+      launch_pair<2, 16, true>(a, tiles, s, batch);
     }
   } else {
     switch (vx) {
-      case 1: launch_pair<1, 16, false>(a, tiles, s); break;
-      case 2: launch_pair<2, 16, false>(a, tiles, s); break;
-      default: launch_pair<4, 12, false>(a, tiles, s); break;
+      case 1: launch_pair<1, 16, false>(a, tiles, s, batch); break;
+      case 2: launch_pair<2, 16, false>(a, tiles, s, batch); break;
+      default: launch_pair<4, 12, false>(a, tiles, s, batch); break;
     }
   }
   check_launch("fgp_pair");
@@ -976,10 +1001,11 @@ void fgp_pair(const float* b, const float* p_k, const float* p_km1, float* p_k1,
 
 namespace {
 
-// Floats of one dual buffer (three components), rounded up to 256 bytes so that every buffer
-// of a workspace starts 256-byte aligned.
-std::size_t dual_floats(int nz, int ny, int nx) {
-  const std::size_t three = 3 * static_cast<std::size_t>(nz) * ny * nx;
+// Floats of one dual buffer (three components per volume of the batch, one volume's after
+// another's), rounded up to 256 bytes so that every buffer of a workspace starts 256-byte
+// aligned.
+std::size_t dual_floats(int nz, int ny, int nx, int batch = 1) {
+  const std::size_t three = 3 * static_cast<std::size_t>(nz) * ny * nx * batch;
   return (three + 63) / 64 * 64;
 }
 
@@ -987,10 +1013,11 @@ std::size_t dual_floats(int nz, int ny, int nx) {
 
 std::size_t TvState::workspace_floats(int nz, int ny, int nx) { return 4 * dual_floats(nz, ny, nx); }
 
-TvState::TvState(int nz, int ny, int nx)
-    : nz_(nz), ny_(ny), nx_(nx), vol_(static_cast<long>(nz) * ny * nx),
-      owned_(workspace_floats(nz, ny, nx)) {
-  const std::size_t stride = dual_floats(nz, ny, nx);
+TvState::TvState(int nz, int ny, int nx, int batch)
+    : nz_(nz), ny_(ny), nx_(nx), vol_(static_cast<long>(nz) * ny * nx), cap_(batch),
+      owned_(4 * dual_floats(nz, ny, nx, std::max(batch, 1))) {
+  require(batch >= 1, "tv: the batch must be positive");
+  const std::size_t stride = dual_floats(nz, ny, nx, batch);
   for (int i = 0; i < 4; ++i) p_[i] = owned_.get() + i * stride;
 }
 
@@ -1000,8 +1027,10 @@ TvState::TvState(int nz, int ny, int nx, float* workspace)
   for (int i = 0; i < 4; ++i) p_[i] = workspace + i * stride;
 }
 
-void TvState::reset(hipStream_t s) {
-  fill(p_[0], 3 * vol_, 0.0f, s);
+void TvState::reset(hipStream_t s, int batch) {
+  require(batch >= 1 && batch <= cap_, "tv: batch beyond the state's capacity");
+  batch_ = batch;
+  fill(p_[0], 3 * vol_ * batch, 0.0f, s);
   cur_ = 0;
 }
 
@@ -1047,7 +1076,7 @@ void TvState::prox(const float* b, float* out, float lambda, int inner, hipStrea
       // The last pass skips p_{k+1}: the next prox starts from p alone.
       const bool last = k + 2 == inner;
       fgp_pair(b, p_[cur], p_[prev], last ? nullptr : p_[f1], p_[f2],
-               nz_, ny_, nx_, lambda, step, m1, k == 0, m2, s);
+               nz_, ny_, nx_, lambda, step, m1, k == 0, m2, s, batch_);
       tn = fista_t(t);
       m_prev = (t - 1.0f) / tn;
       t = tn;
@@ -1056,7 +1085,7 @@ void TvState::prox(const float* b, float* out, float lambda, int inner, hipStrea
       k += 2;
     } else {
       fgp_step(b, p_[cur], p_[prev], p_[f1], nz_, ny_, nx_, lambda, step,
-               m_prev, k == 0, s);
+               m_prev, k == 0, s, batch_);
       const float tn = fista_t(t);
       m_prev = (t - 1.0f) / tn;
       t = tn;
@@ -1068,7 +1097,7 @@ void TvState::prox(const float* b, float* out, float lambda, int inner, hipStrea
   }
   if (t1) t1->record(s);
   cur_ = cur;
-  tv_primal(b, p_[cur_], nz_, ny_, nx_, lambda, out, to_index, nm2, inv_k02, s);
+  tv_primal(b, p_[cur_], nz_, ny_, nx_, lambda, out, to_index, nm2, inv_k02, s, batch_);
 }
 
 }  // namespace qh
