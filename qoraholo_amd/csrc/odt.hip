@@ -203,8 +203,9 @@ OdtSolver::OdtSolver(const OdtGrid& g, int count, const OdtConfig& c, int batch)
     hits_.resize(nb * half_);
     e_.resize(nb * half_);
     x_[0].resize(nb * vol_);
-    x_[1].resize(nb * vol_);
     if (tv_) {
+      // FISTA's previous and prox-input iterates (Gerchberg-Papoulis updates x_[0] in place).
+      x_[1].resize(nb * vol_);
       x_[2].resize(nb * vol_);
       tv_state_ = std::make_unique<TvState>(g.nz, g.n, g.n, batch);
       fgp_events_.resize(2 * c.iterations);
