@@ -439,7 +439,7 @@ int qh_hip_tv_prox(const float* b, int nz, int ny, int nx, float lambda, int inn
 /* qh_hip_tv_prox of `batch` volumes at once (no counterpart in the reference, whose tv_prox
  * takes one): b and x hold the volumes one after another, every FGP pass and the final primal
  * run as one launch over all of them, and each volume gets the bits qh_hip_tv_prox gives it
- * alone. The batched form the served loop's batched slots use. */
+ * alone; batch is 1 to 65535. The batched form the served loop's batched slots use. */
 int qh_hip_tv_prox_batch(const float* b, int batch, int nz, int ny, int nx, float lambda,
                          int inner, int fused, float* x);
 

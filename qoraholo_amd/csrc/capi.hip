@@ -1077,8 +1077,9 @@ int qh_hip_tv_prox(const float* b, int nz, int ny, int nx, float lambda, int inn
 int qh_hip_tv_prox_batch(const float* b, int batch, int nz, int ny, int nx, float lambda,
                          int inner, int fused, float* x) {
   return guarded([&] {
-    require(b && x && batch >= 1 && nz >= 1 && ny >= 1 && nx >= 1 && lambda > 0 && inner >= 0,
-            "tv_prox_batch: bad arguments");
+    require(b && x && batch >= 1 && batch <= 65535 && nz >= 1 && ny >= 1 && nx >= 1 &&
+                lambda > 0 && inner >= 0,
+            "tv_prox_batch: bad arguments (batch 1 to 65535)");
     require_device();
     const hipStream_t s = call_stream();
     const std::size_t vol = static_cast<std::size_t>(nz) * ny * nx;

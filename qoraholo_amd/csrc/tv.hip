@@ -858,7 +858,7 @@ void launch_primal(const TvArgs& a, int chunks, dim3 grid, float* out, int ti, f
 void fgp_step(const float* b, const float* p_in, const float* p_prev, float* p_out, int nz,
               int ny, int nx, float lambda, float step, float momentum, bool first,
               hipStream_t s, int batch) {
-  require(batch >= 1, "fgp_step: batch must be positive");
+  require(batch >= 1 && batch <= 65535, "fgp_step: batch must be 1 to 65535");
   TvArgs a{b, p_in, p_prev, p_out, static_cast<long>(nz) * ny * nx, nz, ny, nx,
            lambda, step, momentum, first ? 1 : 0};
   int chunks = 0;
@@ -875,7 +875,7 @@ void fgp_step(const float* b, const float* p_in, const float* p_prev, float* p_o
 
 void tv_primal(const float* b, const float* p, int nz, int ny, int nx, float lambda, float* out,
                bool to_index, float nm2, float inv_k02, hipStream_t s, int batch) {
-  require(batch >= 1, "tv_primal: batch must be positive");
+  require(batch >= 1 && batch <= 65535, "tv_primal: batch must be 1 to 65535");
   TvArgs a{b, p, nullptr, nullptr, static_cast<long>(nz) * ny * nx, nz, ny, nx,
            lambda, 0.0f, 0.0f, 1};
   int chunks = 0;
@@ -967,7 +967,7 @@ void fgp_pair(const float* b, const float* p_k, const float* p_km1, float* p_k1,
               int nz, int ny, int nx, float lambda, float step, float m1, bool first, float m2,
               hipStream_t s, int batch) {
   require(fgp_pair_supported(nx), "fgp_pair: rows up to 512 wide");
-  require(batch >= 1, "fgp_pair: batch must be positive");
+  require(batch >= 1 && batch <= 65535, "fgp_pair: batch must be 1 to 65535");
   int vx = std::min(lanes_x(nx), 4);
   int tiles_x = 1;
   int xt = nx;
@@ -1016,7 +1016,7 @@ std::size_t TvState::workspace_floats(int nz, int ny, int nx) { return 4 * dual_
 TvState::TvState(int nz, int ny, int nx, int batch)
     : nz_(nz), ny_(ny), nx_(nx), vol_(static_cast<long>(nz) * ny * nx), cap_(batch),
       owned_(4 * dual_floats(nz, ny, nx, std::max(batch, 1))) {
-  require(batch >= 1, "tv: the batch must be positive");
+  require(batch >= 1 && batch <= 65535, "tv: the batch must be 1 to 65535");
   const std::size_t stride = dual_floats(nz, ny, nx, batch);
   for (int i = 0; i < 4; ++i) p_[i] = owned_.get() + i * stride;
 }

@@ -168,6 +168,21 @@ def test_bpm_argument_checks(qh):
         qh.bpm_reconstruct(fields[:2], il, g, ri)
     with pytest.raises(ValueError):
         qh.bpm_vjp(ri - np.float32(1.337), fields[:2], il, g)
+    with pytest.raises(ValueError, match="checkpoint at least -1"):
+        qh.bpm_forward(ri, il, g, checkpoint=-2)
+
+
+@pytest.mark.parametrize("batch", [0, 65536])
+def test_tv_prox_batch_argument_checks(qh, batch):
+    """The batched prox refuses batch sizes outside 1..65535 (the grid's second dimension)
+    before any device work, as ValueError."""
+    from qoraholo_amd import _native
+
+    b = np.zeros((2, 4, 4, 4), np.float32)
+    x = np.empty_like(b)
+    with pytest.raises(ValueError, match="batch 1 to 65535"):
+        _native.check(_native.lib().qh_hip_tv_prox_batch(_native.ptr(b), batch, 4, 4, 4, 0.1, 2, 1,
+                                                         _native.ptr(x)))
 
 
 def test_host_copy_threads_any_size_and_alignment():
