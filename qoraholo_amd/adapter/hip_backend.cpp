@@ -42,9 +42,11 @@ qh_bpm_table_view view_of(const BpmTables& t) {
           t.lipschitz};
 }
 
-// Illuminations per pass chosen by the library (0) and a checkpoint every 16 slices; the
-// tables carry the obliquity.
-qh_bpm_options bpm_layout(int chunk) { return {1, chunk, 16}; }
+// Illuminations per pass chosen by the library (0) and the automatic checkpoint interval (-1):
+// every envelope kept, as src/cuda/bpm.cu stores them, when a pass's illuminations fit within
+// half the free device memory, else a checkpoint every 16 slices; the tables carry the
+// obliquity.
+qh_bpm_options bpm_layout(int chunk) { return {1, chunk, -1}; }
 
 class HipPipeline final : public Pipeline::Impl {
  public:
