@@ -317,8 +317,8 @@ __global__ void __launch_bounds__(64) k_fgp_step(TvArgs a, int chunks) {
 // value is formed with the float operations of k_fgp_step, so a pass equals two single steps
 // bit for bit, while HBM sees 7 volumes read and 6 written per two steps instead of 20.
 // Planes per block: at most 64 and at least 4, chosen per volume by pair_chunk; each chunk
-// starts two planes early, so longer chunks read fewer warm-up planes (384^3 pass 675 -> 637 us
-// from 32 to 64 planes).
+// starts two planes early, so longer chunks read fewer warm-up planes (384^3 pass with x tiles
+// of 2 columns per lane 675 -> 637 us from 32 to 64 planes).
 constexpr int kPairChunk = 64;
 constexpr int kPairChunkMin = 4;
 
@@ -908,8 +908,10 @@ int pair_tile_vx(int nx) { return nx % 4 == 0 ? 4 : 2; }
 // Planes per z chunk for a grid of `tiles` row tiles: the length from kPairChunkMin to
 // kPairChunk that minimises the rounds of blocks each CU runs times the planes a block marches
 // (its chunk and two warm-up planes), the longest on ties. 256^3: 32 planes, 256 blocks, one
-// round; 384^3: 64 planes, three rounds; 128^3: 6 planes and 242 blocks (8-plane chunks left 80
-// CUs idle and spent a fifth of each block on warm-up).
+// round; 384^3 in x tiles of 4 columns per lane: 48 planes, 768 blocks, three full rounds (64
+// planes would leave the third round a quarter full; tiles of 2 columns per lane took 64);
+// 128^3: 6 planes and 242 blocks (8-plane chunks left 80 CUs idle and spent a fifth of each
+// block on warm-up).
 int pair_chunk(const void* kernel, std::atomic<int>& per_cu_cache, int threads, std::size_t lds,
                int tiles, int nz) {
   int per_cu = per_cu_cache.load(std::memory_order_relaxed);

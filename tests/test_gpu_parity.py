@@ -202,7 +202,7 @@ def test_tv_two_step_passes_equal_single_steps(device, shape, inner):
     chunks, odd inner counts (a single last step), row widths for 1, 2 and 4 floats per lane
     with and without whole-vector access, the full 256^3 volume, a small volume (short z
     chunks), rows wider than 256 cut into x tiles of 4 columns per lane (384, 400, 260, 512)
-    or 2 (302, and 505 with element-wise access), the full 384^3 volume (64-plane chunks) and
+    or 2 (302, and 505 with element-wise access), the full 384^3 volume (48-plane chunks) and
     128^3 (6-plane chunks)."""
     from qoraholo_amd import _native
 
