@@ -41,6 +41,8 @@ struct TvArgs {
 
 // Volume blockIdx.y of a batch of volumes that lie one after another: its primal data and
 // output start blockIdx.y vol floats in, its duals (three components each) 3 blockIdx.y vol in.
+// Kernels apply the offsets only in their BATCH instantiations, which single volumes never run:
+// the offsets cost the 256^3 two-step pass 1.3% (154.7 -> 156.8 us) when always applied.
 __device__ __forceinline__ long batch_volume() { return static_cast<long>(blockIdx.y); }
 template <class T>
 __device__ __forceinline__ T* shifted(T* p, long by) {
@@ -225,12 +227,14 @@ __device__ __forceinline__ void load_plane(const TvArgs& a, int z, int y0, int x
   extrapolate<VX>(a, pk, pm, ry_top);
 }
 
-template <int VX, int TY, bool VEC>
+template <int VX, int TY, bool VEC, bool BATCH>
 __global__ void __launch_bounds__(64) k_fgp_step(TvArgs a, int chunks) {
-  a.b = shifted(a.b, batch_volume() * a.vol);
-  a.p_in = shifted(a.p_in, 3 * batch_volume() * a.vol);
-  a.p_prev = shifted(a.p_prev, 3 * batch_volume() * a.vol);
-  a.p_out = shifted(a.p_out, 3 * batch_volume() * a.vol);
+  if constexpr (BATCH) {
+    a.b = shifted(a.b, batch_volume() * a.vol);
+    a.p_in = shifted(a.p_in, 3 * batch_volume() * a.vol);
+    a.p_prev = shifted(a.p_prev, 3 * batch_volume() * a.vol);
+    a.p_out = shifted(a.p_out, 3 * batch_volume() * a.vol);
+  }
   const int t = xcd_tile();
   const int y0 = (t / chunks) * TY;
   const int z0 = (t % chunks) * a.zlen;
@@ -490,13 +494,15 @@ __device__ __forceinline__ void lds_get(const float* buf, int row, int nx, int x
 }
 
 // VEC: see load_row. XT: rows are cut into x tiles (more than one per row).
-template <int VX, int NW, bool VEC, bool XT>
+template <int VX, int NW, bool VEC, bool XT, bool BATCH>
 __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
-  a.b = shifted(a.b, batch_volume() * a.vol);
-  a.p_k = shifted(a.p_k, 3 * batch_volume() * a.vol);
-  a.p_km1 = shifted(a.p_km1, 3 * batch_volume() * a.vol);
-  a.p_k1 = shifted(a.p_k1, 3 * batch_volume() * a.vol);
-  a.p_k2 = shifted(a.p_k2, 3 * batch_volume() * a.vol);
+  if constexpr (BATCH) {
+    a.b = shifted(a.b, batch_volume() * a.vol);
+    a.p_k = shifted(a.p_k, 3 * batch_volume() * a.vol);
+    a.p_km1 = shifted(a.p_km1, 3 * batch_volume() * a.vol);
+    a.p_k1 = shifted(a.p_k1, 3 * batch_volume() * a.vol);
+    a.p_k2 = shifted(a.p_k2, 3 * batch_volume() * a.vol);
+  }
   constexpr int TOP = 2;  // halo rows above the output rows (and two below)
   constexpr int T = NW - 4;
   // LDS row width: the row when a wave holds it whole, else the columns one wave holds.
@@ -712,12 +718,14 @@ __global__ void __launch_bounds__(NW * 64) k_fgp_pair(PairArgs a, int chunks) {
 }
 
 // out = max(b - lambda L(p), 0) (optionally its refractive index), same strip layout.
-template <int VX, int TY, bool VEC>
+template <int VX, int TY, bool VEC, bool BATCH>
 __global__ void __launch_bounds__(64) k_tv_primal(TvArgs a, int chunks, float* out, int to_index,
                                                   float nm2, float inv_k02) {
-  a.b = shifted(a.b, batch_volume() * a.vol);
-  a.p_in = shifted(a.p_in, 3 * batch_volume() * a.vol);
-  out += batch_volume() * a.vol;
+  if constexpr (BATCH) {
+    a.b = shifted(a.b, batch_volume() * a.vol);
+    a.p_in = shifted(a.p_in, 3 * batch_volume() * a.vol);
+    out += batch_volume() * a.vol;
+  }
   const int t = xcd_tile();
   const int y0 = (t / chunks) * TY;
   const int z0 = (t % chunks) * a.zlen;
@@ -828,29 +836,49 @@ bool rows_vec(int nx) {
   return VX % 2 == 0 && nx % VX == 0;
 }
 
-template <int VX>
-void launch_step(const TvArgs& a, int chunks, dim3 grid, hipStream_t s) {
+template <int VX, bool BATCH>
+void launch_step_as(const TvArgs& a, int chunks, dim3 grid, hipStream_t s) {
   if constexpr (VX % 2 == 0) {
     if (rows_vec<VX>(a.nx)) {
-      hipLaunchKernelGGL((k_fgp_step<VX, kRows, true>), grid, dim3(64), 0, s, a, chunks);
+      hipLaunchKernelGGL((k_fgp_step<VX, kRows, true, BATCH>), grid, dim3(64), 0, s, a, chunks);
       return;
     }
   }
-  hipLaunchKernelGGL((k_fgp_step<VX, kRows, false>), grid, dim3(64), 0, s, a, chunks);
+  hipLaunchKernelGGL((k_fgp_step<VX, kRows, false, BATCH>), grid, dim3(64), 0, s, a, chunks);
+}
+
+// A grid with a second dimension runs a batch of volumes (strip_grid).
+template <int VX>
+void launch_step(const TvArgs& a, int chunks, dim3 grid, hipStream_t s) {
+  if (grid.y > 1) {
+    launch_step_as<VX, true>(a, chunks, grid, s);
+  } else {
+    launch_step_as<VX, false>(a, chunks, grid, s);
+  }
+}
+
+template <int VX, bool BATCH>
+void launch_primal_as(const TvArgs& a, int chunks, dim3 grid, float* out, int ti, float nm2,
+                      float inv_k02, hipStream_t s) {
+  if constexpr (VX % 2 == 0) {
+    if (rows_vec<VX>(a.nx)) {
+      hipLaunchKernelGGL((k_tv_primal<VX, kRows, true, BATCH>), grid, dim3(64), 0, s, a, chunks,
+                         out, ti, nm2, inv_k02);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_tv_primal<VX, kRows, false, BATCH>), grid, dim3(64), 0, s, a, chunks, out,
+                     ti, nm2, inv_k02);
 }
 
 template <int VX>
 void launch_primal(const TvArgs& a, int chunks, dim3 grid, float* out, int ti, float nm2,
                    float inv_k02, hipStream_t s) {
-  if constexpr (VX % 2 == 0) {
-    if (rows_vec<VX>(a.nx)) {
-      hipLaunchKernelGGL((k_tv_primal<VX, kRows, true>), grid, dim3(64), 0, s, a, chunks, out, ti,
-                         nm2, inv_k02);
-      return;
-    }
+  if (grid.y > 1) {
+    launch_primal_as<VX, true>(a, chunks, grid, out, ti, nm2, inv_k02, s);
+  } else {
+    launch_primal_as<VX, false>(a, chunks, grid, out, ti, nm2, inv_k02, s);
   }
-  hipLaunchKernelGGL((k_tv_primal<VX, kRows, false>), grid, dim3(64), 0, s, a, chunks, out, ti,
-                     nm2, inv_k02);
 }
 
 }  // namespace
@@ -936,16 +964,25 @@ int pair_chunk(const void* kernel, std::atomic<int>& per_cu_cache, int threads, 
 }
 
 // Volumes of a batch are the grid's second dimension; their blocks count in the chunk choice.
-template <int VX, int NW, bool VEC, bool XT>
-void launch_pair_kernel(PairArgs a, int tiles, std::size_t lds, hipStream_t s, int batch) {
+template <int VX, int NW, bool VEC, bool XT, bool BATCH>
+void launch_pair_kernel_as(PairArgs a, int tiles, std::size_t lds, hipStream_t s, int batch) {
   // Blocks per CU of this kernel (its registers and LDS; the same on every gfx950 device).
   static std::atomic<int> per_cu{0};
-  a.zlen = pair_chunk(reinterpret_cast<const void*>(&k_fgp_pair<VX, NW, VEC, XT>), per_cu,
+  a.zlen = pair_chunk(reinterpret_cast<const void*>(&k_fgp_pair<VX, NW, VEC, XT, BATCH>), per_cu,
                       NW * 64, lds, tiles * batch, a.nz);
   const int chunks = (a.nz + a.zlen - 1) / a.zlen;
-  hipLaunchKernelGGL((k_fgp_pair<VX, NW, VEC, XT>),
+  hipLaunchKernelGGL((k_fgp_pair<VX, NW, VEC, XT, BATCH>),
                      dim3(static_cast<unsigned>(tiles * chunks), static_cast<unsigned>(batch)),
                      dim3(NW * 64), lds, s, a, chunks);
+}
+
+template <int VX, int NW, bool VEC, bool XT>
+void launch_pair_kernel(PairArgs a, int tiles, std::size_t lds, hipStream_t s, int batch) {
+  if (batch > 1) {
+    launch_pair_kernel_as<VX, NW, VEC, XT, true>(a, tiles, lds, s, batch);
+  } else {
+    launch_pair_kernel_as<VX, NW, VEC, XT, false>(a, tiles, lds, s, batch);
+  }
 }
 
 template <int VX, int NW, bool XT>
