@@ -109,6 +109,7 @@ struct BpmModel::Args {
   int red_chunk;
   int red_blocks;
   int red_init;
+  int line_blocks;  // with red_blocks: the grid's first line_blocks blocks run the lines
 };
 
 namespace {
@@ -318,15 +319,16 @@ __device__ __forceinline__ void reduce_terms(const float* p, int chunk, long pla
 // b = q U (src/common.hpp:168-176); inverse y transform of b.
 template <class E>
 __global__ void __launch_bounds__(E::kBlock) k_bpm_cols_adj(Args a) {
-  if (a.red_blocks > 0 && static_cast<long>(blockIdx.x) >= a.g.blocks) {
+  const int nb = a.red_blocks > 0 ? a.line_blocks : static_cast<int>(gridDim.x);
+  if (static_cast<int>(blockIdx.x) >= nb) {
     // A reduction block (the last red_blocks of the grid, dealt to the XCDs in turn after the
     // line blocks): the previous slice's gradient terms, one plane element per thread.
     const long plane = static_cast<long>(a.g.n) * a.g.n;
-    const long i = (static_cast<long>(blockIdx.x) - a.g.blocks) * E::kBlock + threadIdx.x;
+    const long i = static_cast<long>(static_cast<int>(blockIdx.x) - nb) * E::kBlock + threadIdx.x;
     if (i < plane) reduce_terms(a.red_p, a.red_chunk, plane, a.red_grad, a.red_init != 0, i);
     return;
   }
-  E e(a.g);
+  E e(a.g, nb);
   const int n = a.g.n;
   const long plane = static_cast<long>(n) * n;
   e.load([&](long l, int i) { return a.in[col_at<E>(l, i, n, plane)]; });
@@ -463,7 +465,7 @@ void launch_with(Kind k, Args a, unsigned grid, std::size_t lds, hipStream_t s) 
   } else {
     if (k == Kind::cols_adj && a.red_grad != nullptr) {
       const long plane = static_cast<long>(a.g.n) * a.g.n;
-      a.g.blocks = static_cast<int>(grid);
+      a.line_blocks = static_cast<int>(grid);
       a.red_blocks = static_cast<int>((plane + E::kBlock - 1) / E::kBlock);
       grid += static_cast<unsigned>(a.red_blocks);
     }

@@ -345,8 +345,9 @@ __global__ void __launch_bounds__((Shape4<N1, N2, LINE_FAST>::BLOCK)) k_fft4s_li
   }
 }
 
-// z lines of the half spectrum: forward transform, restore, inverse transform.
-template <int N1, int N2>
+// z lines of the half spectrum: forward transform, restore, inverse transform. BATCH: the
+// lines of several volumes (the line offsets' divisions are left out of single volumes).
+template <int N1, int N2, bool BATCH>
 __global__ void __launch_bounds__((Shape4<N1, N2, true>::BLOCK)) k_fft4s_zrt(ZArgs a) {
   using S = Shape4<N1, N2, true>;
   __shared__ float2 tw[S::N];
@@ -357,7 +358,7 @@ __global__ void __launch_bounds__((Shape4<N1, N2, true>::BLOCK)) k_fft4s_zrt(ZAr
   const int j = tid / S::LINES;
   const long l = xcd_block() * S::LINES + line;
   const bool valid = l < a.lines;
-  const long lb = z_line_base(a, l);
+  const long lb = BATCH ? z_line_base(a, l) : l;
   float2 v[N2];
   float2 w[N1];
 #pragma unroll
@@ -572,7 +573,11 @@ struct LaunchZrt {
   static void run(const ZArgs& z, hipStream_t s) {
     using S = Shape4<N1, N2, true>;
     const unsigned grid = static_cast<unsigned>((z.lines + S::LINES - 1) / S::LINES);
-    hipLaunchKernelGGL((k_fft4s_zrt<N1, N2>), dim3(grid), dim3(S::BLOCK), 0, s, z);
+    if (z.vol != 0) {
+      hipLaunchKernelGGL((k_fft4s_zrt<N1, N2, true>), dim3(grid), dim3(S::BLOCK), 0, s, z);
+    } else {
+      hipLaunchKernelGGL((k_fft4s_zrt<N1, N2, false>), dim3(grid), dim3(S::BLOCK), 0, s, z);
+    }
   }
 };
 

@@ -19,14 +19,19 @@ inline constexpr int kMaxFactors = 16;
 // neighbouring blocks that share 128-byte lines (line batches whose rows are not line aligned,
 // byte arrays read 16 bytes per row) share them through one L2 instead of fetching them once
 // per XCD. The last nb mod 8 blocks keep their numbers.
-// With nb given, the renumbering covers the first nb blocks of the grid only (the later blocks
-// run other work and must not call it).
-__device__ __forceinline__ long xcd_block(long nb) {
+__device__ __forceinline__ long xcd_block() {
+  const long nb = gridDim.x;
   const long b = blockIdx.x;
   const long full = nb - nb % 8;
   return b < full ? (b % 8) * (full / 8) + b / 8 : b;
 }
-__device__ __forceinline__ long xcd_block() { return xcd_block(static_cast<long>(gridDim.x)); }
+// The same renumbering over the grid's first nb blocks only (the later blocks run other work
+// and must not call it).
+__device__ __forceinline__ long xcd_block(int nb) {
+  const int b = static_cast<int>(blockIdx.x);
+  const int full = nb - nb % 8;
+  return b < full ? (b % 8) * (full / 8) + b / 8 : b;
+}
 
 struct FftFactors {
   int n = 0;

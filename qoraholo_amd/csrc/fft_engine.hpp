@@ -40,7 +40,6 @@ struct LineGeom {
   int n;
   long lines;
   int lines_per_block;  // LdsLines only
-  int blocks;           // the first `blocks` blocks of the grid run the lines (0: every block)
 };
 
 constexpr int ce_max(int a, int b) { return a > b ? a : b; }
@@ -86,7 +85,13 @@ struct RegLines {
   static unsigned grid(long lines) { return static_cast<unsigned>((lines + LINES - 1) / LINES); }
   static std::size_t dynamic_lds(int) { return 0; }
 
-  __device__ explicit RegLines(const LineGeom& g) {
+  __device__ explicit RegLines(const LineGeom& g) { init<false>(g, 0); }
+  // The grid's first nb blocks run the lines (its later blocks run other work).
+  __device__ RegLines(const LineGeom& g, int nb) { init<true>(g, nb); }
+  // The block's line index is formed after the twiddle fill has been issued, so that the fill's
+  // loads do not wait for the grid size (a short kernel's start is on its critical path).
+  template <bool PART>
+  __device__ void init(const LineGeom& g, int nb) {
     t = lds_t();
     tw = lds_tw();
     const int tid = static_cast<int>(threadIdx.x);
@@ -97,7 +102,7 @@ struct RegLines {
     }
     line = LINE_FAST ? tid % LINES : tid / T;
     j = LINE_FAST ? tid / LINES : tid % T;
-    l = xcd_block(g.blocks > 0 ? g.blocks : static_cast<long>(gridDim.x)) * LINES + line;
+    l = (PART ? xcd_block(nb) : xcd_block()) * LINES + line;
     valid = l < g.lines;
   }
 
@@ -200,7 +205,11 @@ struct LdsLines {
            sizeof(float2);
   }
 
-  __device__ explicit LdsLines(const LineGeom& g) {
+  __device__ explicit LdsLines(const LineGeom& g) { init<false>(g, 0); }
+  // The grid's first nb blocks run the lines (its later blocks run other work).
+  __device__ LdsLines(const LineGeom& g, int nb) { init<true>(g, nb); }
+  template <bool PART>
+  __device__ void init(const LineGeom& g, int nb) {
     extern __shared__ float2 smem[];
     n = g.n;
     f = &g.f;
@@ -209,7 +218,7 @@ struct LdsLines {
     b1 = b0 + g.lines_per_block * n;
     cur = b0;
     for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = g.tw[i];
-    l0 = xcd_block(g.blocks > 0 ? g.blocks : static_cast<long>(gridDim.x)) * g.lines_per_block;
+    l0 = (PART ? xcd_block(nb) : xcd_block()) * g.lines_per_block;
     const long left = g.lines - l0;
     nl = static_cast<int>(left < g.lines_per_block ? left : g.lines_per_block);
     ls = CONTIG ? n : 1;
