@@ -295,6 +295,35 @@ def test_full_size_invariants(device, qh):
     torch.cuda.synchronize()
 
 
+def test_automatic_checkpoint_with_little_free_memory(device, qh):
+    """256^2 x 256 with 49 illuminations while all but 8 GiB of the device memory is taken: the
+    envelopes (6.6 GB) would not fit within half the free memory, so the automatic interval
+    falls back to checkpoints every 16 slices, still in one pass of all 49 illuminations, and the
+    adjoint has the bits of a plan that keeps every envelope."""
+    torch = pytest.importorskip("torch")
+    g = qh.Geometry(*GEOM)
+    il = qh.circular_scan(49, 0.9, g)
+    n = nz = 256
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    x = 0.02 * torch.rand((nz, n, n), generator=gen, device="cuda")
+    r = torch.randn((49, n, n), dtype=torch.complex64, generator=gen, device="cuda")
+    torch.cuda.synchronize()
+    free, _ = torch.cuda.mem_get_info()
+    print(f"bpm automatic checkpoint: {free / 2**30:.1f} GiB free, keeping 8", flush=True)
+    assert free > 16 << 30
+    hog = torch.empty(free - (8 << 30), dtype=torch.uint8, device="cuda")
+    plan = qh.BpmPlan(nz, n, il, g)
+    assert plan.checkpoint == 16 and plan.chunk == 49
+    g16 = plan.vjp(x, r)
+    torch.cuda.synchronize()
+    del plan, hog
+    torch.cuda.empty_cache()
+    stored = qh.BpmPlan(nz, n, il, g, checkpoint=0)
+    assert torch.equal(stored.vjp(x, r), g16)
+    del stored
+    torch.cuda.synchronize()
+
+
 def overlapping_beads(ph):
     """Three overlapping beads, radii 2-4 um, Delta n up to 0.048 above the medium."""
     return [ph.Sphere(0.0, 0.0, 0.0, 4.0, 1.385), ph.Sphere(2.5, 1.5, -1.0, 3.0, 1.372),
