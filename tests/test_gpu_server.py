@@ -217,13 +217,15 @@ def test_tight_budget_evicts_across_request_kinds(device, qh, mixed, bpm_mixed):
         assert s["device_bytes"] <= budget and s["slots"] == 1
 
 
-@pytest.mark.parametrize("solver,iterations", [("gp", 5), ("tv", 3)])
-def test_batched_slots_give_each_request_its_alone_result(device, qh, solver, iterations):
+@pytest.mark.parametrize("solver,iterations,holo", [("gp", 5, 128), ("tv", 3, 128), ("gp", 4, 256),
+                                                  ("tv", 2, 256)])
+def test_batched_slots_give_each_request_its_alone_result(device, qh, solver, iterations, holo):
     """A geometry with batched slots: queued requests of it run together (one front end for
-    all their stacks, one graph replay per batch) and each gets the bits it gets alone on a
-    Pipeline, whatever batch it lands in; distinct inputs show the results are routed to their
-    own requests."""
-    d = PipelineData(qh, holo=128, angles=9, iterations=iterations, solver=solver)
+    all their stacks, one graph replay per batch, every solve launch over the batch's volumes)
+    and each gets the bits it gets alone on a Pipeline, whatever batch it lands in; distinct
+    inputs show the results are routed to their own requests. 64^3 and 128^3 volumes (z round
+    trips of 64 and 128 points, FGP rows of 1 and 2 floats per lane)."""
+    d = PipelineData(qh, holo=holo, angles=9, iterations=iterations, solver=solver)
     inputs = [(d.holo * np.float32(1.0 + 0.02 * k)).astype(np.float32) for k in range(3)]
     p = qh.Pipeline(d.count, d.size, illuminations=d.illum, geometry=d.geom, **d.kw)
     p.set_background(d.bg)
