@@ -172,6 +172,19 @@ def test_bpm_argument_checks(qh):
         qh.bpm_forward(ri, il, g, checkpoint=-2)
 
 
+def test_bpm_plan_checkpoint_needs_a_plan(qh):
+    """The resolved checkpoint interval is read from a plan; a null plan is refused before any
+    device work, as ValueError."""
+    import ctypes
+
+    from qoraholo_amd import _native
+
+    out = ctypes.c_int(-5)
+    with pytest.raises(ValueError, match="null plan"):
+        _native.check(_native.lib().qh_hip_bpm_plan_checkpoint(None, ctypes.byref(out)))
+    assert out.value == -5
+
+
 @pytest.mark.parametrize("batch", [0, 65536])
 def test_tv_prox_batch_argument_checks(qh, batch):
     """The batched prox refuses batch sizes outside 1..65535 (the grid's second dimension)
