@@ -202,7 +202,7 @@ def cpu_baseline(w, runs=3):
 
 
 def streaming_reference(torch, mib=256, reps=20):
-    """What plain streaming kernels reach on this box, beside the roofline's 8 TB/s: torch's
+    """What plain streaming kernels reach on this device, beside the roofline's 8 TB/s: torch's
     device copy (read 1, write 1) and its vectorised addcmul (read 3 float4 streams, write 1) on
     `mib` MiB float32 tensors, timed with events over `reps` launches each."""
     n = mib * 1024 * 1024 // 4
@@ -896,7 +896,7 @@ def main() -> int:
         roofline["traffic_frac"] = round(roofline["traffic_gbs"] / HBM_PEAK_GBS, 4)
     roofline["streaming_reference"] = streaming_reference(torch)
     if roofline.get("traffic_gbs"):
-        # The measured traffic rate against what plain streaming kernels reach on this box in
+        # The measured traffic rate against what plain streaming kernels reach on this device in
         # this run (the peak in `frac` is the datasheet's).
         ref = roofline["streaming_reference"]
         roofline["traffic_vs_copy"] = round(roofline["traffic_gbs"] / ref["copy_gbs"], 4)
