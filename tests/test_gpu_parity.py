@@ -218,24 +218,25 @@ def test_tv_two_step_passes_equal_single_steps(device, shape, inner):
     assert np.array_equal(outs[0], outs[1])
 
 
-@pytest.mark.parametrize("shape,inner", [((9, 20, 64), 5), ((16, 16, 128), 4),
-                                         ((20, 30, 300), 3), ((12, 18, 302), 5),
-                                         ((40, 64, 256), 4)])
-def test_tv_batch_equals_volumes_alone(device, shape, inner):
-    """A batch of three volumes proxed in shared launches (the grid's second dimension) gives
-    each volume the bits it gets alone: untiled rows of 1, 2 and 4 floats per lane, x tiles of 4
-    and 2 columns per lane, odd inner counts (a last single step) and the final primal, two-step
-    and single-step passes."""
+@pytest.mark.parametrize("shape,inner,batch", [((9, 20, 64), 5, 3), ((16, 16, 128), 4, 3),
+                                               ((20, 30, 300), 3, 3), ((12, 18, 302), 5, 3),
+                                               ((40, 64, 256), 4, 3), ((256, 256, 256), 4, 2)])
+def test_tv_batch_equals_volumes_alone(device, shape, inner, batch):
+    """A batch of volumes proxed in shared launches (the grid's second dimension) gives each
+    volume the bits it gets alone: untiled rows of 1, 2 and 4 floats per lane, x tiles of 4 and
+    2 columns per lane, odd inner counts (a last single step) and the final primal, two-step and
+    single-step passes, and two full 256^3 volumes, whose batch marches longer z chunks than a
+    volume alone (64 planes against 32)."""
     from qoraholo_amd import _native
 
     rng = np.random.default_rng(shape[2])
-    b = (rng.standard_normal((3,) + shape) * 0.1 + 0.05).astype(np.float32)
-    print(f"tv batch shape={shape} inner={inner}", flush=True)
+    b = (rng.standard_normal((batch,) + shape) * 0.1 + 0.05).astype(np.float32)
+    print(f"tv batch {batch} shape={shape} inner={inner}", flush=True)
     for fused in (1, 0):
         got = np.empty_like(b)
-        _native.check(_native.lib().qh_hip_tv_prox_batch(_native.ptr(b), 3, *shape, 0.02, inner,
-                                                         fused, _native.ptr(got)))
-        for k in range(3):
+        _native.check(_native.lib().qh_hip_tv_prox_batch(_native.ptr(b), batch, *shape, 0.02,
+                                                         inner, fused, _native.ptr(got)))
+        for k in range(batch):
             alone = np.empty_like(b[k])
             _native.check(_native.lib().qh_hip_tv_prox(_native.ptr(b[k]), *shape, 0.02, inner,
                                                        fused, _native.ptr(alone)))
